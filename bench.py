@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Throughput of the 6D wave-energy step on one MI355X.
+
+Workload (default): the production 6D grid nx=ny=nz=24, nk=nl=nm=16
+(56,623,104 cells) under N(z) = 5.2e-3 exp(z/1300) s^-1, U(z) = 0.2
+tanh((z+500)/300) m/s, a Garrett-Munk-shaped spectrum with a seeded lognormal
+(x,y) modulation, dt at CFL 0.5, binary64.  A "step" is one integrate
+(for_src/integrate.f90:4-271) over the whole grid with E resident in HBM.
+
+Prints ONE JSON line (rank 0).  ``value`` is 6D cell-updates/s of the plain
+binary64 step as the library runs it on this input; ``roofline`` is that step
+kernel's algorithmic HBM bytes per launch over its event-timed average launch
+duration against the 8 TB/s MI355X peak; ``cpu_baseline`` times the reference
+Fortran (oracle/_ref, MPI ranks on the host cores) on a bounded sample of the
+same workload.  Extra keys report the FP32 form and the general (6D
+coefficient fields in HBM) path.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _time_steps(m, dt, steps, warmup):
+    """W untimed steps, then EXACTLY K timed steps bracketed by device
+    synchronisation; returns (wall seconds, event ms of the K launches)."""
+    if warmup:
+        m.integrate(warmup, dt=dt)
+    m.synchronize()
+    t0 = time.perf_counter()
+    ev_ms = m.integrate_timed(steps, dt=dt)
+    m.synchronize()
+    t1 = time.perf_counter()
+    return t1 - t0, ev_ms
+
+
+def _pe_grid(n, npes):
+    """A legal PE grid for the reference decomposition (parallel_mpi.f90:14-28):
+    factors of 2 over x, y (any split) and k, l (even splits only)."""
+    grid = [1] * 6
+    order = [0, 1, 3, 4, 2, 5]
+    left = npes
+    while left > 1 and left % 2 == 0:
+        placed = False
+        for ax in order:
+            nb = grid[ax] * 2
+            if n[ax] % nb == 0 and n[ax] // nb >= 2:
+                grid[ax] = nb
+                left //= 2
+                placed = True
+                break
+        if not placed:
+            break
+    return grid if left == 1 else None
+
+
+def cpu_baseline(case, dt, steps):
+    """The reference Fortran (oracle/_ref/ref_harness) on the host cores,
+    MPI ranks over a 6D PE grid; returns the cpu_baseline object."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from raytracer_3d_amd import cases
+
+    if not os.path.exists(O.REF_BIN):
+        return None
+    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    tries = []
+    for npes in (16, 8, 4, 2):
+        if npes <= ncpu:
+            g = _pe_grid(case.n, npes)
+            if g:
+                tries.append((npes, g))
+    tries.append((1, None))
+    with tempfile.TemporaryDirectory() as tmp:
+        cf = os.path.join(tmp, "bench.case")
+        cases.write_case_file(case, cf, nsteps=steps, dt=dt)
+        last_err = None
+        for npes, g in tries:
+            try:
+                _log(f"cpu baseline: reference Fortran, {npes} MPI rank(s), grid {g}, {steps} steps")
+                secs = O.run_reference(cf, "-", np_grid=g, timeout=900)
+                return {
+                    "value": case.cells * steps / secs,
+                    "unit": "cell-updates/s",
+                    "cores": npes,
+                    "kind": "reference",
+                    "sample": f"{steps} integrate steps of the same 24^3x16^3 binary64 workload, "
+                              f"MPI_Wtime around the loop, {npes} rank(s) PE grid {g}, host nproc {ncpu}",
+                    "seconds": secs,
+                }
+            except Exception as e:  # noqa: BLE001 -- try the next rank count
+                last_err = str(e)[-300:]
+                _log(f"cpu baseline with {npes} ranks failed: {last_err}")
+    return {"value": None, "unit": "cell-updates/s", "cores": 0, "kind": "reference",
+            "sample": "failed: " + (last_err or "")}
+
+
+def _traffic(workload, kernel):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            t = json.load(f)
+        e = t.get(workload, {}).get(kernel)
+        return e["bytes_per_launch"] if e else None
+    except Exception:  # noqa: BLE001
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: the 6D step runs one resident grid per GPU and this build has "
+              "no multi-GPU path (see DESIGN.md for how it would shard); run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from raytracer_3d_amd import cases, rt6d
+
+    _log("building the 24^3 x 16^3 workload")
+    case = cases.gm_case()
+    cells = case.cells
+    workload = "gm_24x24x24_16x16x16"
+
+    def run(precision, path):
+        m = rt6d.Model.from_case(case, precision=precision, path=path)
+        m.calc_parameter()
+        dt = m.cfl_dt(0.5)
+        wall, ev_ms = _time_steps(m, dt, args.steps, args.warmup)
+        res = {
+            "value": cells * args.steps / wall,
+            "ms_per_step": 1e3 * wall / args.steps,
+            "launch_ms": ev_ms / args.steps,
+            "bytes_per_cell": m.bytes_per_cell,
+            "homogeneous": m.homogeneous,
+            "dt": dt,
+        }
+        res["achieved_GBs"] = res["bytes_per_cell"] * cells / (res["launch_ms"] * 1e-3) / 1e9
+        m.close()
+        return res
+
+    _log("binary64 step")
+    main_res = run(rt6d.FP64, 0)
+    extras = {}
+    if not args.no_extras:
+        _log("binary64 step, general path")
+        extras["fp64_general"] = run(rt6d.FP64, 1)
+        _log("FP32 form")
+        extras["fp32"] = run(rt6d.FP32, 0)
+        extras["fp32"]["speedup_vs_fp64"] = extras["fp32"]["value"] / main_res["value"]
+        extras["fp32_general"] = run(rt6d.FP32, 1)
+
+    kname = "k_step<double, false, true>" if main_res["homogeneous"] else "k_step<double, false, false>"
+    achieved = main_res["achieved_GBs"]
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": _traffic(workload, kname),
+        "kernel": kname,
+        "algorithmic_bytes_per_cell": main_res["bytes_per_cell"],
+        "launch_ms": round(main_res["launch_ms"], 4),
+    }
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(case, main_res["dt"], args.cpu_steps)
+
+    out = {
+        "metric": "6D cell-updates/s FP64 & FP32 at 24^3x16^3; batched p50/p99 latency",
+        "value": main_res["value"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": main_res["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": workload, "n": list(case.n), "cells": cells, "cfl": 0.5,
+                   "dt": main_res["dt"], "boundaries": "open x,y; reflecting z (reference semantics)",
+                   "path": "homogeneous" if main_res["homogeneous"] else "general"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if extras:
+        out["extras"] = extras
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,168 @@
+/*
+ * rt6d.h -- C ABI of the MI355X (gfx950) 6D wave-energy time step.
+ *
+ * The library holds one resident device copy of the reference model's state
+ * (for_src/main_module.f90:41-77): E, the group velocities and refraction
+ * rates, the mean-flow growth rate, the 1D grids and the background N^2, U, V.
+ * Every entry point below stands in for one reference routine or for the
+ * module state it works on; the reference routines are argument-less Fortran
+ * subroutines over main_module globals, so the arguments here are exactly the
+ * globals each one reads or writes.
+ *
+ * Conventions
+ *  - Every call returns 0 on success and a negative RT6D_E* code on failure;
+ *    rt6d_last_error() then gives a message (per host thread).  Nothing
+ *    aborts; the Fortran bindings turn a non-zero status into halt_stop, as
+ *    the reference does on fatal errors (for_src/parallel_mpi.f90:223-236).
+ *  - Host arrays are borrowed for the duration of a call only (copied in or
+ *    out).  Device buffers are owned by the context and carry no halos:
+ *    ghost cells are index arithmetic inside the kernels.
+ *  - Arrays are Fortran order (x fastest) over (x,y,z,k,l,m); axis numbers
+ *    0..5 follow that order.  "Halo" layouts span (1-2 : n+2) per axis, the
+ *    reference's onx=2 allocation (for_src/main_module.f90:114,135-136).
+ *  - One context is used by one host thread at a time; calls on one context
+ *    are serialised on the context's own HIP stream.
+ */
+#ifndef RT6D_H
+#define RT6D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RT6D_OK 0
+#define RT6D_EINVAL (-1)   /* bad argument or call order */
+#define RT6D_EHIP (-2)     /* HIP runtime error */
+#define RT6D_ENOMEM (-3)   /* device allocation failed */
+#define RT6D_ESTATE (-4)   /* required setup step missing */
+
+/* precision of the step */
+#define RT6D_FP64 0        /* E, fluxes and velocities in binary64 */
+#define RT6D_FP32 1        /* E, fluxes, velocities in binary32; coefficients
+                              derived in binary64, energy sums compensated binary64 */
+
+/* field ids for rt6d_download */
+#define RT6D_FIELD_E 0
+#define RT6D_FIELD_DE 1
+#define RT6D_FIELD_MEANFL 2
+#define RT6D_FIELD_OMEGA_I 3
+#define RT6D_FIELD_OMEGA_E 4
+#define RT6D_FIELD_XDOT 5    /* 5..10: xdot ydot zdot kdot ldot mdot */
+#define RT6D_FIELD_FLUX_X 11 /* 11..16: flux_x .. flux_m */
+#define RT6D_FIELD_LAMBDA_X 17 /* 17..19: lambda_x lambda_y lambda_z */
+#define RT6D_FIELD_LAMBDA_SUM 20 /* lambda summed over active x,y,z */
+
+/* layouts for host arrays */
+#define RT6D_LAYOUT_INTERIOR 0 /* n_x*...*n_m */
+#define RT6D_LAYOUT_FACES 1    /* velocities/fluxes: 0..n_d along their own axis */
+#define RT6D_LAYOUT_HALO 2     /* (n+4) per axis, onx = 2 */
+
+/* options for rt6d_set_option */
+#define RT6D_OPT_EXACT 1     /* 1: reference operation order, IEEE divides, no FMA.
+                                0 (default): reciprocals, division-free limiter, FMA */
+#define RT6D_OPT_PATH 2      /* 0 auto (default), 1 force the general path
+                                (6D coefficient fields streamed from HBM) */
+#define RT6D_OPT_BLOCK 3     /* threads per block of the step kernel (64..1024) */
+
+/* queries for rt6d_query */
+#define RT6D_Q_HOMOGENEOUS 1 /* 1 if the horizontally homogeneous path is active */
+#define RT6D_Q_DEVICE_BYTES 2
+#define RT6D_Q_CELLS 3
+#define RT6D_Q_STEPS 4       /* steps taken since the last upload of E */
+#define RT6D_Q_BYTES_PER_CELL 5 /* algorithmic HBM bytes per cell-update of the active step kernel */
+
+typedef struct rt6d_ctx rt6d_ctx;
+
+/* Scalars of diagnose.f90:45-329 (no netCDF). */
+typedef struct rt6d_diag {
+    double eint;       /* sum dE * vol */
+    double mflint;     /* sum meanfl * vol */
+    double mfl_max;    /* max |meanfl| / max(1e-12, E) */
+    double vmax[6];    /* umax vmax wmax kmax lmax mmax */
+    double bfl[12];    /* bfl_xs bfl_xe bfl_ys bfl_ye bfl_zs bfl_ze bfl_ks bfl_ke bfl_ls bfl_le bfl_ms bfl_me */
+    double cfl[6];     /* cfl_x .. cfl_m */
+    double residual;   /* eint + mflint + sum(bfl_e - bfl_s) */
+} rt6d_diag;
+
+/* Library build/version string. */
+const char *rt6d_version(void);
+
+/* Message for the last failing call on this host thread. */
+const char *rt6d_last_error(void);
+
+/* Number of visible HIP devices (0 when none); never fails. */
+int rt6d_device_count(void);
+
+/*
+ * allocate_main_module (for_src/main_module.f90:131-238) + the switches of
+ * set_parameter (main_module.f90:24-31).  n[6] = nx ny nz nk nl nm, each >= 1;
+ * precision RT6D_FP64 or RT6D_FP32.  Uses the current HIP device.
+ */
+int rt6d_create(const int *n, int reflect_up, int reflect_down, int precision, rt6d_ctx **out);
+int rt6d_destroy(rt6d_ctx *ctx);
+
+/*
+ * Host helper restating calc_grid for one axis (for_src/calc_grid.f90:104-177,
+ * u_centered_grid :300-322).  spacing: n interior T spacings.  axis 0..5.
+ * fixed: k_fixed/l_fixed/m_fixed for a wavenumber axis with n = 1.
+ * Outputs t, dt, u, du: n+4 values each (halo'd), any may be NULL.
+ */
+int rt6d_build_grid(int axis, int n, const double *spacing, double fixed,
+                    double *t, double *dt, double *u, double *du);
+
+/* The 1D grid of one axis as calc_grid leaves it: T positions and T spacings,
+ * n+4 values each (halo'd).  Replaces main_module's xt/dxt .. mt/dmt. */
+int rt6d_set_grid(rt6d_ctx *ctx, int axis, const double *t, const double *dt);
+
+/* set_initial_conditions' background: Nsqr, U, V as halo'd 3D arrays
+ * ((nx+4)(ny+4)(nz+4)), f0 and sign_of_omega_i (main_module.f90:24-26,62-64). */
+int rt6d_set_background(rt6d_ctx *ctx, const double *Nsqr, const double *U, const double *V,
+                        double f0, double sign_of_omega_i);
+
+/* calc_parameter (for_src/calc_parameter.f90:2-264) on the device. */
+int rt6d_calc_parameter(rt6d_ctx *ctx);
+
+/* dt = cfl / max_cell sum_d max(|v_d(c-1/2)|, |v_d(c+1/2)|) / dxt_d(c), over
+ * active axes, reduced in binary64 on the device.  (No reference counterpart:
+ * the reference takes dt from its config; diagnose.f90:294-313 only prints CFL.) */
+int rt6d_cfl_dt(rt6d_ctx *ctx, double cfl, double *dt);
+
+/* Set E (main_module.f90:41); layout INTERIOR or HALO (ghosts ignored, as
+ * integrate refills them every step). */
+int rt6d_upload_E(rt6d_ctx *ctx, const double *E, int layout);
+
+/* integrate (for_src/integrate.f90:4-271) called nsteps times with step dt. */
+int rt6d_step(rt6d_ctx *ctx, double dt, int nsteps);
+
+/* As rt6d_step; *ms receives the device time of the nsteps launches, from HIP
+ * events recorded on the context's stream around them. */
+int rt6d_step_timed(rt6d_ctx *ctx, double dt, int nsteps, double *ms);
+
+/* superbee_fluxes (for_src/superbee_fluxes.f90:4-174) on the current E, with
+ * integrate's ghost treatment; results readable with rt6d_download(FLUX_*). */
+int rt6d_fluxes(rt6d_ctx *ctx, double dt);
+
+/* Copy a field to the host.  E: current state.  DE, MEANFL, FLUX_*: those of
+ * the last integrate (as the reference leaves them after integrate), or of
+ * the last rt6d_fluxes call if it came later.  Velocities/lambdas/omegas:
+ * calc_parameter outputs.  layout INTERIOR, FACES (velocities, fluxes) or
+ * HALO (E only; ghosts set as integrate sets them). */
+int rt6d_download(rt6d_ctx *ctx, int field, int layout, double *out);
+
+/*
+ * diagnose reductions (for_src/diagnose.f90:45-329) for the state after the
+ * last step: E is the current state, dE/meanfl/fluxes those of that step.
+ * 3D outputs (nx*ny*nz each, may be NULL): E_int, meanfl_int, and diss6 =
+ * 6 consecutive arrays diss_ks diss_ke diss_ls diss_le diss_ms diss_me.
+ */
+int rt6d_diagnose(rt6d_ctx *ctx, double dt, rt6d_diag *out,
+                  double *E_int, double *meanfl_int, double *diss6);
+
+int rt6d_set_option(rt6d_ctx *ctx, int option, int value);
+int rt6d_query(rt6d_ctx *ctx, int what, long long *value);
+int rt6d_synchronize(rt6d_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RT6D_H */

@@ -1,0 +1,943 @@
+// rt6d_api.cpp -- C ABI of the 6D step (include/rt6d.h): context, device
+// memory, call ordering and dispatch to the kernels of rt6d_exact.hip and
+// rt6d_fast.hip.  Host-side only; every numerical operation on 6D data runs
+// on the GPU.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rt6d.h"
+#include "rt6d_kernels.h"
+
+using namespace rt6d;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) return fail(RT6D_EHIP, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+Geo make_geo(const int *n)
+{
+    Geo g{};
+    uint64_t cells = 1;
+    for (int d = 0; d < kAxes; ++d) {
+        g.n[d] = n[d];
+        g.inner[d] = (int)cells;
+        cells *= (uint64_t)n[d];
+        if (n[d] > 1) g.act |= 1 << d;
+    }
+    for (int d = 0; d < 5; ++d) g.div[d] = make_fastdiv((uint32_t)n[d]);
+    g.ncells = (uint32_t)cells;
+    return g;
+}
+
+}  // namespace
+
+struct rt6d_ctx {
+    int n[kAxes] = {0};
+    int up = 1, down = 1, prec = RT6D_FP64;
+    size_t esz = 8;               // bytes per element of the step precision
+    Geo g{};
+    uint64_t cells = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double f0 = 1e-4, sign = 1.0;
+    std::vector<double> t_h[kAxes], dt_h[kAxes];
+    bool grid_set[kAxes] = {false, false, false, false, false, false};
+    bool bg_set = false;
+    // device buffers
+    double *d_grid = nullptr;     // per axis: t (n+4) then dt (n+4)
+    size_t grid_off[kAxes] = {0};
+    double *d_bg = nullptr;       // Nsqr, U, V halo'd 3D
+    size_t n3h = 0;
+    double *d_coef = nullptr;     // per axis: dx, rdx, cfl (n+1 each)
+    size_t coef_off[kAxes] = {0};
+    double coef_dt = NAN;
+    void *d_E[2] = {nullptr, nullptr};
+    int cur = 0;
+    bool E_set = false, have_prev = false;
+    double prev_dt = 0.0;
+    long long steps = 0;
+    void *d_vel[kAxes] = {nullptr};
+    void *d_lam = nullptr;
+    size_t vel_count[kAxes] = {0};
+    bool params_ready = false;
+    bool homog = false;
+    int opt_exact = 0, opt_path = 0, opt_block = 256;
+    void *d_hx = nullptr, *d_hy = nullptr, *d_hz = nullptr, *d_hm = nullptr, *d_hl = nullptr;
+    uint32_t n4 = 0, inner4m = 0;
+    double vmax[kAxes] = {0};
+    double cflsum_max = 0.0;
+    void *d_tdE = nullptr, *d_tmf = nullptr, *d_tfl[kAxes] = {nullptr};
+    int fl_src = 0, de_src = 0;   // 0 stale, 1 from the last step's start state, 2 from current E
+    double *d_partial = nullptr;
+    uint32_t nparts = 0;
+    double *d_dout = nullptr;
+    double *d_d3 = nullptr;       // E_int, meanfl_int, 6 diss arrays
+    size_t dev_bytes = 0;
+    int *d_flag = nullptr;
+    unsigned long long *d_bits = nullptr;
+
+    int dalloc(void **p, size_t bytes)
+    {
+        hipError_t e = hipMalloc(p, bytes ? bytes : 8);
+        if (e != hipSuccess) {
+            *p = nullptr;
+            return fail(RT6D_ENOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+        }
+        dev_bytes += bytes;
+        return 0;
+    }
+    void dfree(void *&p)
+    {
+        if (p) hipFree(p);
+        p = nullptr;
+    }
+};
+
+namespace {
+
+int check_ctx(rt6d_ctx *c)
+{
+    if (!c) return fail(RT6D_EINVAL, "null context");
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return fail(RT6D_EHIP, "hipSetDevice: %s", hipGetErrorString(e));
+    return 0;
+}
+
+// dx, rdx and dt/dx of every axis for step dt (binary64, on the device)
+int ensure_coef(rt6d_ctx *c, double dt)
+{
+    if (c->coef_dt == dt) return 0;
+    size_t tot = 0;
+    for (int d = 0; d < kAxes; ++d) tot += 3 * (size_t)(c->n[d] + 1);
+    std::vector<double> h(tot);
+    for (int d = 0; d < kAxes; ++d) {
+        const int n = c->n[d];
+        double *dx = h.data() + c->coef_off[d];
+        double *rdx = dx + (n + 1);
+        double *cf = rdx + (n + 1);
+        for (int f = 0; f <= n; ++f) {
+            dx[f] = c->dt_h[d][f + 1];  // Fortran index f -> halo slot f+1
+            rdx[f] = 1.0 / dx[f];
+            cf[f] = dt / dx[f];
+        }
+    }
+    HIPCHK(hipMemcpyAsync(c->d_coef, h.data(), tot * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->coef_dt = dt;
+    return 0;
+}
+
+AxisCoef axis_coef(const rt6d_ctx *c)
+{
+    AxisCoef a{};
+    for (int d = 0; d < kAxes; ++d) {
+        a.dx[d] = c->d_coef + c->coef_off[d];
+        a.rdx[d] = a.dx[d] + (c->n[d] + 1);
+        a.cfl[d] = a.rdx[d] + (c->n[d] + 1);
+    }
+    return a;
+}
+
+ParamIn param_in(const rt6d_ctx *c)
+{
+    ParamIn pi{};
+    for (int d = 0; d < kAxes; ++d) {
+        pi.t[d] = c->d_grid + c->grid_off[d];
+        pi.dt[d] = pi.t[d] + (c->n[d] + 4);
+        pi.n[d] = c->n[d];
+    }
+    pi.Nsqr = c->d_bg;
+    pi.U = c->d_bg + c->n3h;
+    pi.V = c->d_bg + 2 * c->n3h;
+    pi.f0 = c->f0;
+    pi.sign = c->sign;
+    pi.up = c->up;
+    pi.down = c->down;
+    return pi;
+}
+
+template <typename T>
+StepPtrs<T> step_ptrs(const rt6d_ctx *c, const void *Ein, void *Eout, double dt)
+{
+    StepPtrs<T> p{};
+    p.Ein = (const T *)Ein;
+    p.Eout = (T *)Eout;
+    for (int d = 0; d < kAxes; ++d) p.vel[d] = (const T *)c->d_vel[d];
+    p.lam = (const T *)c->d_lam;
+    p.hom.vx3 = (const T *)c->d_hx;
+    p.hom.vy3 = (const T *)c->d_hy;
+    p.hom.vz = (const T *)c->d_hz;
+    p.hom.vm = (const T *)c->d_hm;
+    p.hom.lam = (const T *)c->d_hl;
+    p.hom.n4 = c->n4;
+    p.hom.inner4m = c->inner4m;
+    p.ax = axis_coef(c);
+    p.dt = dt;
+    p.dE = (T *)c->d_tdE;
+    p.meanfl = (T *)c->d_tmf;
+    for (int d = 0; d < kAxes; ++d) p.flux[d] = (T *)c->d_tfl[d];
+    return p;
+}
+
+int require_ready(rt6d_ctx *c)
+{
+    if (!c->params_ready) return fail(RT6D_ESTATE, "calc_parameter has not run");
+    if (!c->E_set) return fail(RT6D_ESTATE, "E has not been uploaded");
+    return 0;
+}
+
+int ensure_tendency_buffers(rt6d_ctx *c)
+{
+    if (c->d_tdE) return 0;
+    int r;
+    if ((r = c->dalloc(&c->d_tdE, c->cells * c->esz))) return r;
+    if ((r = c->dalloc(&c->d_tmf, c->cells * c->esz))) return r;
+    for (int d = 0; d < kAxes; ++d) {
+        if ((r = c->dalloc(&c->d_tfl[d], c->vel_count[d] * c->esz))) return r;
+        HIPCHK(hipMemsetAsync(c->d_tfl[d], 0, c->vel_count[d] * c->esz, c->stream));
+    }
+    HIPCHK(hipMemsetAsync(c->d_tdE, 0, c->cells * c->esz, c->stream));
+    HIPCHK(hipMemsetAsync(c->d_tmf, 0, c->cells * c->esz, c->stream));
+    return 0;
+}
+
+// Tendency (dE, meanfl, fluxes) of E buffer `which` with step dt.
+int run_tendency(rt6d_ctx *c, const void *E, double dt)
+{
+    int r;
+    if ((r = ensure_tendency_buffers(c))) return r;
+    if ((r = ensure_coef(c, dt))) return r;
+    hipError_t e;
+    if (c->prec == RT6D_FP64) {
+        StepPtrs<double> p = step_ptrs<double>(c, E, nullptr, dt);
+        e = c->opt_exact ? launch_tendency_exact(c->g, p, c->homog, c->stream)
+                         : launch_tendency_fast<double>(c->g, p, c->homog, c->stream);
+    } else {
+        StepPtrs<float> p = step_ptrs<float>(c, E, nullptr, dt);
+        e = launch_tendency_fast<float>(c->g, p, c->homog, c->stream);
+    }
+    HIPCHK(e);
+    return 0;
+}
+
+// dE/meanfl/fluxes as integrate leaves them after the last step
+int ensure_step_byproducts(rt6d_ctx *c, bool need_flux)
+{
+    if (c->de_src == 1 && (!need_flux || c->fl_src != 0)) return 0;
+    int r;
+    if ((r = ensure_tendency_buffers(c))) return r;
+    if (!c->have_prev) {
+        // no step yet: the reference's zero-initialised arrays
+        HIPCHK(hipMemsetAsync(c->d_tdE, 0, c->cells * c->esz, c->stream));
+        HIPCHK(hipMemsetAsync(c->d_tmf, 0, c->cells * c->esz, c->stream));
+        for (int d = 0; d < kAxes; ++d)
+            HIPCHK(hipMemsetAsync(c->d_tfl[d], 0, c->vel_count[d] * c->esz, c->stream));
+    } else {
+        if ((r = run_tendency(c, c->d_E[c->cur ^ 1], c->prev_dt))) return r;
+    }
+    c->de_src = 1;
+    c->fl_src = 1;
+    return 0;
+}
+
+int download_typed(rt6d_ctx *c, const void *src, size_t count, double *dst)
+{
+    if (c->prec == RT6D_FP64) {
+        HIPCHK(hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } else {
+        std::vector<float> tmp(count);
+        HIPCHK(hipMemcpyAsync(tmp.data(), src, count * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (size_t i = 0; i < count; ++i) dst[i] = (double)tmp[i];
+    }
+    return 0;
+}
+
+// copy an interior-shaped array into a halo'd host array (interior part only)
+void interior_to_halo(const int *n, const double *in, double *out)
+{
+    const int H0 = n[0] + 4, H1 = n[1] + 4, H2 = n[2] + 4, H3 = n[3] + 4, H4 = n[4] + 4;
+    size_t q = 0;
+    for (int m = 0; m < n[5]; ++m)
+        for (int l5 = 0; l5 < n[4]; ++l5)
+            for (int l = 0; l < n[3]; ++l)
+                for (int k = 0; k < n[2]; ++k)
+                    for (int j = 0; j < n[1]; ++j) {
+                        size_t base = (size_t)2 + H0 * ((size_t)(j + 2) + H1 * ((size_t)(k + 2) +
+                                      H2 * ((size_t)(l + 2) + H3 * ((size_t)(l5 + 2) + (size_t)H4 * (m + 2)))));
+                        memcpy(out + base, in + q, (size_t)n[0] * sizeof(double));
+                        q += n[0];
+                    }
+}
+
+void halo_to_interior(const int *n, const double *in, double *out)
+{
+    const int H0 = n[0] + 4, H1 = n[1] + 4, H2 = n[2] + 4, H3 = n[3] + 4, H4 = n[4] + 4;
+    size_t q = 0;
+    for (int m = 0; m < n[5]; ++m)
+        for (int l5 = 0; l5 < n[4]; ++l5)
+            for (int l = 0; l < n[3]; ++l)
+                for (int k = 0; k < n[2]; ++k)
+                    for (int j = 0; j < n[1]; ++j) {
+                        size_t base = (size_t)2 + H0 * ((size_t)(j + 2) + H1 * ((size_t)(k + 2) +
+                                      H2 * ((size_t)(l + 2) + H3 * ((size_t)(l5 + 2) + (size_t)H4 * (m + 2)))));
+                        memcpy(out + q, in + base, (size_t)n[0] * sizeof(double));
+                        q += n[0];
+                    }
+}
+
+// Faces 1..n_d of a face array (the reference's T-point values) -> interior.
+void faces_to_interior(const int *n, int d, const double *in, double *out)
+{
+    size_t inner = 1, outer = 1;
+    for (int e = 0; e < d; ++e) inner *= (size_t)n[e];
+    for (int e = d + 1; e < kAxes; ++e) outer *= (size_t)n[e];
+    const size_t nd = (size_t)n[d];
+    for (size_t o = 0; o < outer; ++o)
+        for (size_t f = 1; f <= nd; ++f)
+            memcpy(out + (o * nd + (f - 1)) * inner, in + (o * (nd + 1) + f) * inner, inner * sizeof(double));
+}
+
+}  // namespace
+
+// ====================================================================== ABI
+
+extern "C" {
+
+const char *rt6d_version(void) { return "rt6d 0.1 gfx950"; }
+
+const char *rt6d_last_error(void) { return g_err.c_str(); }
+
+int rt6d_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int rt6d_build_grid(int axis, int n, const double *spacing, double fixed, double *t, double *dt, double *u,
+                    double *du)
+{
+    if (axis < 0 || axis >= kAxes || n < 1 || !spacing) return fail(RT6D_EINVAL, "rt6d_build_grid: bad arguments");
+    const int N = n + 4;
+    std::vector<double> D(N), Du(N), Tt(N), Uu(N);
+    // Fortran index f in [-1, n+2] at slot f+1; ghosts take the edge spacing (calc_grid.f90:104-111)
+    for (int f = 1; f <= n; ++f) D[f + 1] = spacing[f - 1];
+    for (int i = 1; i <= 2; ++i) {
+        D[n + i + 1] = D[n + 1];
+        D[1 - i + 1] = D[2];
+    }
+    // u_centered_grid (calc_grid.f90:300-322) over the halo'd array
+    Uu[0] = 0.0;
+    for (int p = 1; p < N; ++p) Uu[p] = Uu[p - 1] + D[p];
+    Tt[0] = Uu[0] - D[0] * 0.5;
+    for (int p = 1; p < N; ++p) Tt[p] = 2.0 * Uu[p - 1] - Tt[p - 1];
+    for (int p = 0; p < N - 1; ++p) Du[p] = Tt[p + 1] - Tt[p];
+    Du[N - 1] = 2.0 * D[N - 1] - Du[(N - 2) > 0 ? N - 2 : 0];
+    double sh;
+    if (axis <= 1) {
+        sh = Du[2];           // dxu(1), calc_grid.f90:115,124
+    } else if (axis == 2) {
+        sh = Uu[n + 1];       // zu(nz), :133
+    } else if (n > 1) {
+        sh = Uu[n / 2 + 1];   // u(n/2), :143,156,169
+    } else {
+        sh = 0.0;
+    }
+    for (int p = 0; p < N; ++p) {
+        if (axis >= 3 && n == 1) {
+            Tt[p] = fixed;
+            Uu[p] = fixed;
+        } else {
+            Tt[p] = Tt[p] - sh;
+            Uu[p] = Uu[p] - sh;
+        }
+    }
+    if (t) memcpy(t, Tt.data(), N * sizeof(double));
+    if (dt) memcpy(dt, D.data(), N * sizeof(double));
+    if (u) memcpy(u, Uu.data(), N * sizeof(double));
+    if (du) memcpy(du, Du.data(), N * sizeof(double));
+    return 0;
+}
+
+int rt6d_create(const int *n, int reflect_up, int reflect_down, int precision, rt6d_ctx **out)
+{
+    if (!n || !out) return fail(RT6D_EINVAL, "rt6d_create: null argument");
+    *out = nullptr;
+    uint64_t cells = 1;
+    for (int d = 0; d < kAxes; ++d) {
+        if (n[d] < 1) return fail(RT6D_EINVAL, "rt6d_create: n[%d] = %d < 1", d, n[d]);
+        cells *= (uint64_t)n[d];
+    }
+    // face arrays hold cells*(n_d+1)/n_d <= 2*cells elements, indexed in 32 bits
+    if (cells * 2 >= (1ull << 31)) return fail(RT6D_EINVAL, "rt6d_create: %llu cells exceed the 2^30 limit",
+                                               (unsigned long long)cells);
+    if (precision != RT6D_FP64 && precision != RT6D_FP32)
+        return fail(RT6D_EINVAL, "rt6d_create: precision %d", precision);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(RT6D_EHIP, "rt6d_create: no HIP device");
+    rt6d_ctx *c = new rt6d_ctx();
+    for (int d = 0; d < kAxes; ++d) c->n[d] = n[d];
+    c->up = reflect_up ? 1 : 0;
+    c->down = reflect_down ? 1 : 0;
+    c->prec = precision;
+    c->esz = precision == RT6D_FP64 ? 8 : 4;
+    c->cells = cells;
+    c->g = make_geo(n);
+    c->g.up = c->up;
+    c->g.down = c->down;
+    hipGetDevice(&c->device);
+    int r = 0;
+    auto bail = [&](int code) {
+        rt6d_destroy(c);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(RT6D_EHIP, "hipStreamCreate failed"));
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+        return bail(fail(RT6D_EHIP, "hipEventCreate failed"));
+    size_t gtot = 0, ctot = 0;
+    for (int d = 0; d < kAxes; ++d) {
+        c->grid_off[d] = gtot;
+        gtot += 2 * (size_t)(n[d] + 4);
+        c->coef_off[d] = ctot;
+        ctot += 3 * (size_t)(n[d] + 1);
+        c->t_h[d].assign(n[d] + 4, 0.0);
+        c->dt_h[d].assign(n[d] + 4, 0.0);
+        size_t fc = cells / (uint64_t)n[d] * (uint64_t)(n[d] + 1);
+        c->vel_count[d] = fc;
+    }
+    c->n3h = (size_t)(n[0] + 4) * (n[1] + 4) * (n[2] + 4);
+    if ((r = c->dalloc((void **)&c->d_grid, gtot * sizeof(double)))) return bail(r);
+    if ((r = c->dalloc((void **)&c->d_coef, ctot * sizeof(double)))) return bail(r);
+    if ((r = c->dalloc((void **)&c->d_bg, 3 * c->n3h * sizeof(double)))) return bail(r);
+    if ((r = c->dalloc(&c->d_E[0], cells * c->esz))) return bail(r);
+    if ((r = c->dalloc(&c->d_E[1], cells * c->esz))) return bail(r);
+    if ((r = c->dalloc((void **)&c->d_flag, sizeof(int)))) return bail(r);
+    if ((r = c->dalloc((void **)&c->d_bits, 8 * sizeof(unsigned long long)))) return bail(r);
+    if (hipMemsetAsync(c->d_E[0], 0, cells * c->esz, c->stream) != hipSuccess ||
+        hipMemsetAsync(c->d_E[1], 0, cells * c->esz, c->stream) != hipSuccess ||
+        hipMemsetAsync(c->d_bg, 0, 3 * c->n3h * sizeof(double), c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return bail(fail(RT6D_EHIP, "initial clear failed"));
+    c->n4 = (uint32_t)(cells / ((uint64_t)n[0] * n[1]));
+    c->inner4m = (uint32_t)n[2] * n[3] * n[4];
+    *out = c;
+    return 0;
+}
+
+int rt6d_destroy(rt6d_ctx *c)
+{
+    if (!c) return 0;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    c->dfree((void *&)c->d_grid);
+    c->dfree((void *&)c->d_coef);
+    c->dfree((void *&)c->d_bg);
+    c->dfree(c->d_E[0]);
+    c->dfree(c->d_E[1]);
+    for (int d = 0; d < kAxes; ++d) {
+        c->dfree(c->d_vel[d]);
+        c->dfree(c->d_tfl[d]);
+    }
+    c->dfree(c->d_lam);
+    c->dfree(c->d_hx);
+    c->dfree(c->d_hy);
+    c->dfree(c->d_hz);
+    c->dfree(c->d_hm);
+    c->dfree(c->d_hl);
+    c->dfree(c->d_tdE);
+    c->dfree(c->d_tmf);
+    c->dfree((void *&)c->d_partial);
+    c->dfree((void *&)c->d_dout);
+    c->dfree((void *&)c->d_d3);
+    c->dfree((void *&)c->d_flag);
+    c->dfree((void *&)c->d_bits);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+int rt6d_set_grid(rt6d_ctx *c, int axis, const double *t, const double *dt)
+{
+    int r;
+    if ((r = check_ctx(c))) return r;
+    if (axis < 0 || axis >= kAxes || !t || !dt) return fail(RT6D_EINVAL, "rt6d_set_grid: bad arguments");
+    const int N = c->n[axis] + 4;
+    for (int p = 0; p < N; ++p) {
+        if (!(dt[p] > 0.0) || !std::isfinite(dt[p]))
+            return fail(RT6D_EINVAL, "rt6d_set_grid: axis %d spacing[%d] = %g is not positive", axis, p, dt[p]);
+    }
+    memcpy(c->t_h[axis].data(), t, N * sizeof(double));
+    memcpy(c->dt_h[axis].data(), dt, N * sizeof(double));
+    HIPCHK(hipMemcpyAsync(c->d_grid + c->grid_off[axis], t, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_grid + c->grid_off[axis] + N, dt, N * sizeof(double), hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->grid_set[axis] = true;
+    c->coef_dt = NAN;
+    c->params_ready = false;
+    return 0;
+}
+
+int rt6d_set_background(rt6d_ctx *c, const double *Nsqr, const double *U, const double *V, double f0,
+                        double sign_of_omega_i)
+{
+    int r;
+    if ((r = check_ctx(c))) return r;
+    if (!Nsqr || !U || !V) return fail(RT6D_EINVAL, "rt6d_set_background: null array");
+    c->f0 = f0;
+    c->sign = sign_of_omega_i;
+    const size_t b = c->n3h * sizeof(double);
+    HIPCHK(hipMemcpyAsync(c->d_bg, Nsqr, b, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_bg + c->n3h, U, b, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_bg + 2 * c->n3h, V, b, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->bg_set = true;
+    c->params_ready = false;
+    return 0;
+}
+
+int rt6d_calc_parameter(rt6d_ctx *c)
+{
+    int r;
+    if ((r = check_ctx(c))) return r;
+    for (int d = 0; d < kAxes; ++d)
+        if (!c->grid_set[d]) return fail(RT6D_ESTATE, "rt6d_calc_parameter: grid of axis %d not set", d);
+    if (!c->bg_set) return fail(RT6D_ESTATE, "rt6d_calc_parameter: background not set");
+    // release previous coefficient fields
+    for (int d = 0; d < kAxes; ++d) c->dfree(c->d_vel[d]);
+    c->dfree(c->d_lam);
+    c->dfree(c->d_hx);
+    c->dfree(c->d_hy);
+    c->dfree(c->d_hz);
+    c->dfree(c->d_hm);
+    c->dfree(c->d_hl);
+    c->homog = false;
+    c->params_ready = false;
+    if ((r = ensure_coef(c, 1.0))) return r;
+    const ParamIn pi = param_in(c);
+    // binary64 face velocities and lambda
+    double *v64[kAxes] = {nullptr};
+    double *l64 = nullptr;
+    auto cleanup64 = [&]() {
+        for (int d = 0; d < kAxes; ++d)
+            if (v64[d] && (void *)v64[d] != c->d_vel[d]) hipFree(v64[d]);
+        if (l64 && (void *)l64 != c->d_lam) hipFree(l64);
+    };
+    for (int d = 0; d < kAxes; ++d) {
+        if ((r = c->dalloc((void **)&v64[d], c->vel_count[d] * sizeof(double)))) { cleanup64(); return r; }
+        hipError_t e = launch_param_velocity(d, pi, v64[d], c->stream);
+        if (e != hipSuccess) { cleanup64(); return fail(RT6D_EHIP, "param velocity %d: %s", d, hipGetErrorString(e)); }
+    }
+    if ((r = c->dalloc((void **)&l64, c->cells * sizeof(double)))) { cleanup64(); return r; }
+    {
+        hipError_t e = launch_param_lambda(pi, v64, l64, nullptr, c->stream);
+        if (e != hipSuccess) { cleanup64(); return fail(RT6D_EHIP, "param lambda: %s", hipGetErrorString(e)); }
+    }
+    // velocity maxima (diagnose) and the CFL sum (rt6d_cfl_dt)
+    HIPCHK(hipMemsetAsync(c->d_bits, 0, 8 * sizeof(unsigned long long), c->stream));
+    HIPCHK(launch_vmax(c->g, v64, c->d_bits, c->stream));
+    HIPCHK(launch_cfl_max(c->g, v64, axis_coef(c), c->d_bits + 6, c->stream));
+    unsigned long long bits[8];
+    HIPCHK(hipMemcpyAsync(bits, c->d_bits, sizeof bits, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int d = 0; d < kAxes; ++d) memcpy(&c->vmax[d], &bits[d], 8);
+    memcpy(&c->cflsum_max, &bits[6], 8);
+    if (c->prec == RT6D_FP64) {
+        for (int d = 0; d < kAxes; ++d) c->d_vel[d] = v64[d];
+        c->d_lam = l64;
+    } else {
+        for (int d = 0; d < kAxes; ++d) {
+            if ((r = c->dalloc(&c->d_vel[d], c->vel_count[d] * sizeof(float)))) { cleanup64(); return r; }
+            HIPCHK(launch_convert_d2f(v64[d], (float *)c->d_vel[d], c->vel_count[d], c->stream));
+        }
+        if ((r = c->dalloc(&c->d_lam, c->cells * sizeof(float)))) { cleanup64(); return r; }
+        HIPCHK(launch_convert_d2f(l64, (float *)c->d_lam, c->cells, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        cleanup64();
+        for (int d = 0; d < kAxes; ++d) c->dev_bytes -= c->vel_count[d] * sizeof(double);
+        c->dev_bytes -= c->cells * sizeof(double);
+    }
+    // horizontally homogeneous coefficients: tables over (z,k,l,m), checked
+    // bit for bit against every cell before the path is used
+    if (c->opt_path != 1 && c->n[0] * c->n[1] > 1) {
+        const size_t es = c->esz;
+        const size_t nzf = (size_t)c->n4 / c->n[2] * (c->n[2] + 1);
+        const size_t nmf = (size_t)c->inner4m * (c->n[5] + 1);
+        if ((r = c->dalloc(&c->d_hx, 3 * (size_t)c->n4 * es))) return r;
+        if ((r = c->dalloc(&c->d_hy, 3 * (size_t)c->n4 * es))) return r;
+        if ((r = c->dalloc(&c->d_hz, nzf * es))) return r;
+        if ((r = c->dalloc(&c->d_hm, nmf * es))) return r;
+        if ((r = c->dalloc(&c->d_hl, (size_t)c->n4 * es))) return r;
+        HIPCHK(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
+        hipError_t e;
+        if (c->prec == RT6D_FP64) {
+            const double *vv[kAxes];
+            for (int d = 0; d < kAxes; ++d) vv[d] = (const double *)c->d_vel[d];
+            e = launch_build_homog<double>(c->g, vv, (const double *)c->d_lam, (double *)c->d_hx, (double *)c->d_hy,
+                                           (double *)c->d_hz, (double *)c->d_hm, (double *)c->d_hl, c->d_flag,
+                                           c->stream);
+        } else {
+            const float *vv[kAxes];
+            for (int d = 0; d < kAxes; ++d) vv[d] = (const float *)c->d_vel[d];
+            e = launch_build_homog<float>(c->g, vv, (const float *)c->d_lam, (float *)c->d_hx, (float *)c->d_hy,
+                                          (float *)c->d_hz, (float *)c->d_hm, (float *)c->d_hl, c->d_flag, c->stream);
+        }
+        HIPCHK(e);
+        int bad = 1;
+        HIPCHK(hipMemcpyAsync(&bad, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->homog = bad == 0;
+        if (!c->homog) {
+            c->dfree(c->d_hx);
+            c->dfree(c->d_hy);
+            c->dfree(c->d_hz);
+            c->dfree(c->d_hm);
+            c->dfree(c->d_hl);
+        }
+    }
+    c->params_ready = true;
+    c->fl_src = c->de_src = 0;
+    return 0;
+}
+
+int rt6d_cfl_dt(rt6d_ctx *c, double cfl, double *dt)
+{
+    int r;
+    if ((r = check_ctx(c))) return r;
+    if (!dt || !(cfl > 0.0)) return fail(RT6D_EINVAL, "rt6d_cfl_dt: bad arguments");
+    if (!c->params_ready) return fail(RT6D_ESTATE, "rt6d_cfl_dt: calc_parameter has not run");
+    if (!(c->cflsum_max > 0.0)) return fail(RT6D_EINVAL, "rt6d_cfl_dt: all velocities vanish");
+    *dt = cfl / c->cflsum_max;
+    return 0;
+}
+
+int rt6d_upload_E(rt6d_ctx *c, const double *E, int layout)
+{
+    int r;
+    if ((r = check_ctx(c))) return r;
+    if (!E) return fail(RT6D_EINVAL, "rt6d_upload_E: null array");
+    const double *src = E;
+    std::vector<double> tmp;
+    if (layout == RT6D_LAYOUT_HALO) {
+        tmp.resize(c->cells);
+        halo_to_interior(c->n, E, tmp.data());
+        src = tmp.data();
+    } else if (layout != RT6D_LAYOUT_INTERIOR) {
+        return fail(RT6D_EINVAL, "rt6d_upload_E: layout %d", layout);
+    }
+    if (c->prec == RT6D_FP64) {
+        HIPCHK(hipMemcpyAsync(c->d_E[c->cur], src, c->cells * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } else {
+        std::vector<float> f(c->cells);
+        for (size_t i = 0; i < c->cells; ++i) f[i] = (float)src[i];
+        HIPCHK(hipMemcpyAsync(c->d_E[c->cur], f.data(), c->cells * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    c->E_set = true;
+    c->have_prev = false;
+    c->steps = 0;
+    c->fl_src = c->de_src = 0;
+    return 0;
+}
+
+static int do_steps(rt6d_ctx *c, double dt, int nsteps)
+{
+    int r;
+    if ((r = require_ready(c))) return r;
+    if (nsteps < 0 || !std::isfinite(dt)) return fail(RT6D_EINVAL, "rt6d_step: bad arguments");
+    if (nsteps == 0) return 0;
+    if ((r = ensure_coef(c, dt))) return r;
+    const LaunchCfg lc{c->opt_block, c->stream};
+    for (int s = 0; s < nsteps; ++s) {
+        void *in = c->d_E[c->cur], *out = c->d_E[c->cur ^ 1];
+        hipError_t e;
+        if (c->prec == RT6D_FP64) {
+            StepPtrs<double> p = step_ptrs<double>(c, in, out, dt);
+            e = c->opt_exact ? launch_step_exact(c->g, p, c->homog, lc)
+                             : launch_step_fast<double>(c->g, p, c->homog, lc);
+        } else {
+            StepPtrs<float> p = step_ptrs<float>(c, in, out, dt);
+            e = launch_step_fast<float>(c->g, p, c->homog, lc);
+        }
+        HIPCHK(e);
+        c->cur ^= 1;
+    }
+    c->have_prev = true;
+    c->prev_dt = dt;
+    c->steps += nsteps;
+    c->fl_src = c->de_src = 0;
+    return 0;
+}
+
+int rt6d_step(rt6d_ctx *c, double dt, int nsteps)
+{
+    int r;
+    if ((r = check_ctx(c))) return r;
+    if ((r = do_steps(c, dt, nsteps))) return r;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int rt6d_step_timed(rt6d_ctx *c, double dt, int nsteps, double *ms)
+{
+    int r;
+    if ((r = check_ctx(c))) return r;
+    if ((r = require_ready(c))) return r;
+    if ((r = ensure_coef(c, dt))) return r;
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    if ((r = do_steps(c, dt, nsteps))) return r;
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, c->ev0, c->ev1));
+    if (ms) *ms = (double)f;
+    return 0;
+}
+
+int rt6d_fluxes(rt6d_ctx *c, double dt)
+{
+    int r;
+    if ((r = check_ctx(c))) return r;
+    if ((r = require_ready(c))) return r;
+    if ((r = run_tendency(c, c->d_E[c->cur], dt))) return r;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->fl_src = 2;
+    c->de_src = 0;
+    return 0;
+}
+
+int rt6d_download(rt6d_ctx *c, int field, int layout, double *out)
+{
+    int r;
+    if ((r = check_ctx(c))) return r;
+    if (!out) return fail(RT6D_EINVAL, "rt6d_download: null output");
+    const size_t cells = c->cells;
+    if (field == RT6D_FIELD_E) {
+        if (!c->E_set) return fail(RT6D_ESTATE, "rt6d_download: E not uploaded");
+        if (layout == RT6D_LAYOUT_INTERIOR) return download_typed(c, c->d_E[c->cur], cells, out);
+        if (layout == RT6D_LAYOUT_HALO) {
+            std::vector<double> tmp(cells);
+            if ((r = download_typed(c, c->d_E[c->cur], cells, tmp.data()))) return r;
+            interior_to_halo(c->n, tmp.data(), out);
+            return 0;
+        }
+        return fail(RT6D_EINVAL, "rt6d_download: layout %d for E", layout);
+    }
+    if (field == RT6D_FIELD_DE || field == RT6D_FIELD_MEANFL) {
+        if ((r = require_ready(c))) return r;
+        if (layout != RT6D_LAYOUT_INTERIOR) return fail(RT6D_EINVAL, "rt6d_download: dE/meanfl are interior fields");
+        if ((r = ensure_step_byproducts(c, false))) return r;
+        return download_typed(c, field == RT6D_FIELD_DE ? c->d_tdE : c->d_tmf, cells, out);
+    }
+    if (field >= RT6D_FIELD_FLUX_X && field < RT6D_FIELD_FLUX_X + 6) {
+        if ((r = require_ready(c))) return r;
+        const int d = field - RT6D_FIELD_FLUX_X;
+        if (c->fl_src == 0 && (r = ensure_step_byproducts(c, true))) return r;
+        if (layout == RT6D_LAYOUT_FACES) return download_typed(c, c->d_tfl[d], c->vel_count[d], out);
+        if (layout == RT6D_LAYOUT_INTERIOR) {
+            std::vector<double> tmp(c->vel_count[d]);
+            if ((r = download_typed(c, c->d_tfl[d], c->vel_count[d], tmp.data()))) return r;
+            faces_to_interior(c->n, d, tmp.data(), out);
+            return 0;
+        }
+        return fail(RT6D_EINVAL, "rt6d_download: layout %d for a flux", layout);
+    }
+    if (!c->params_ready) return fail(RT6D_ESTATE, "rt6d_download: calc_parameter has not run");
+    if (field >= RT6D_FIELD_XDOT && field < RT6D_FIELD_XDOT + 6) {
+        const int d = field - RT6D_FIELD_XDOT;
+        if (layout == RT6D_LAYOUT_FACES) return download_typed(c, c->d_vel[d], c->vel_count[d], out);
+        if (layout == RT6D_LAYOUT_INTERIOR) {
+            std::vector<double> tmp(c->vel_count[d]);
+            if ((r = download_typed(c, c->d_vel[d], c->vel_count[d], tmp.data()))) return r;
+            faces_to_interior(c->n, d, tmp.data(), out);
+            return 0;
+        }
+        return fail(RT6D_EINVAL, "rt6d_download: layout %d for a velocity", layout);
+    }
+    if (layout != RT6D_LAYOUT_INTERIOR) return fail(RT6D_EINVAL, "rt6d_download: field %d is interior-only", field);
+    if (field == RT6D_FIELD_LAMBDA_SUM) return download_typed(c, c->d_lam, cells, out);
+    const ParamIn pi = param_in(c);
+    if (field >= RT6D_FIELD_LAMBDA_X && field < RT6D_FIELD_LAMBDA_X + 3) {
+        // recompute in binary64 from binary64 velocities
+        double *v64[kAxes] = {nullptr}, *tmp = nullptr;
+        double *lsum = nullptr;
+        hipError_t e = hipSuccess;
+        for (int d = 0; d < 3 && e == hipSuccess; ++d) {
+            e = hipMalloc((void **)&v64[d], c->vel_count[d] * sizeof(double));
+            if (e == hipSuccess) e = launch_param_velocity(d, pi, v64[d], c->stream);
+        }
+        if (e == hipSuccess) e = hipMalloc((void **)&tmp, 3 * cells * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc((void **)&lsum, cells * sizeof(double));
+        if (e == hipSuccess) e = launch_param_lambda(pi, v64, lsum, tmp, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(out, tmp + (size_t)(field - RT6D_FIELD_LAMBDA_X) * cells, cells * sizeof(double),
+                               hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        for (int d = 0; d < 3; ++d) if (v64[d]) hipFree(v64[d]);
+        if (tmp) hipFree(tmp);
+        if (lsum) hipFree(lsum);
+        HIPCHK(e);
+        return 0;
+    }
+    if (field == RT6D_FIELD_OMEGA_I || field == RT6D_FIELD_OMEGA_E) {
+        double *tmp = nullptr;
+        HIPCHK(hipMalloc((void **)&tmp, cells * sizeof(double)));
+        hipError_t e = launch_param_omega(pi, field == RT6D_FIELD_OMEGA_I ? tmp : nullptr,
+                                          field == RT6D_FIELD_OMEGA_E ? tmp : nullptr, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(out, tmp, cells * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        hipFree(tmp);
+        HIPCHK(e);
+        return 0;
+    }
+    return fail(RT6D_EINVAL, "rt6d_download: unknown field %d", field);
+}
+
+int rt6d_diagnose(rt6d_ctx *c, double dt, rt6d_diag *out, double *E_int, double *meanfl_int, double *diss6)
+{
+    int r;
+    if ((r = check_ctx(c))) return r;
+    if ((r = require_ready(c))) return r;
+    if (!out) return fail(RT6D_EINVAL, "rt6d_diagnose: null output");
+    if (!c->have_prev) return fail(RT6D_ESTATE, "rt6d_diagnose: no step has been taken since E was set");
+    const uint32_t nb = (uint32_t)((c->cells + 255) / 256);
+    if (c->nparts < nb) {
+        c->dfree((void *&)c->d_partial);
+        if ((r = c->dalloc((void **)&c->d_partial, (size_t)nb * kDiagSlots * sizeof(double)))) return r;
+        c->nparts = nb;
+    }
+    if (!c->d_dout && (r = c->dalloc((void **)&c->d_dout, 16 * sizeof(double)))) return r;
+    const size_t n3 = (size_t)c->n[0] * c->n[1] * c->n[2];
+    const bool want3 = E_int || meanfl_int || diss6;
+    if (want3 && !c->d_d3 && (r = c->dalloc((void **)&c->d_d3, 8 * n3 * sizeof(double)))) return r;
+    if ((r = ensure_coef(c, c->prev_dt))) return r;
+    const void *Eprev = c->d_E[c->cur ^ 1];
+    const void *Ecur = c->d_E[c->cur];
+    double *Ei = want3 ? c->d_d3 : nullptr, *Mi = want3 ? c->d_d3 + n3 : nullptr,
+           *Di = want3 ? c->d_d3 + 2 * n3 : nullptr;
+    hipError_t e;
+    if (c->prec == RT6D_FP64) {
+        StepPtrs<double> p = step_ptrs<double>(c, Eprev, nullptr, c->prev_dt);
+        e = c->opt_exact ? launch_diag_exact(c->g, p, c->homog, (const double *)Ecur, c->d_partial, c->nparts,
+                                             c->d_dout, Ei, Mi, Di, c->stream)
+                         : launch_diag_fast<double>(c->g, p, c->homog, (const double *)Ecur, c->d_partial,
+                                                    c->nparts, c->d_dout, Ei, Mi, Di, c->stream);
+    } else {
+        StepPtrs<float> p = step_ptrs<float>(c, Eprev, nullptr, c->prev_dt);
+        e = launch_diag_fast<float>(c->g, p, c->homog, (const float *)Ecur, c->d_partial, c->nparts, c->d_dout, Ei,
+                                    Mi, Di, c->stream);
+    }
+    HIPCHK(e);
+    double s[kDiagSlots];
+    HIPCHK(hipMemcpyAsync(s, c->d_dout, sizeof s, hipMemcpyDeviceToHost, c->stream));
+    if (want3) {
+        if (E_int) HIPCHK(hipMemcpyAsync(E_int, Ei, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        if (meanfl_int) HIPCHK(hipMemcpyAsync(meanfl_int, Mi, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        if (diss6) HIPCHK(hipMemcpyAsync(diss6, Di, 6 * n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    memset(out, 0, sizeof *out);
+    out->eint = s[0];
+    out->mflint = s[1];
+    out->mfl_max = s[2];
+    for (int q = 0; q < 12; ++q) out->bfl[q] = s[3 + q];
+    for (int d = 0; d < kAxes; ++d) {
+        out->vmax[d] = c->vmax[d];
+        double m = 0.0;
+        for (int i = 1; i <= c->n[d]; ++i) {
+            const double x = c->vmax[d] * dt / c->dt_h[d][i + 1];
+            m = m > x ? m : x;
+        }
+        out->cfl[d] = m;
+    }
+    const double *b = out->bfl;
+    out->residual = out->eint + out->mflint + b[1] + b[3] + b[5] + b[7] + b[9] + b[11] - b[0] - b[2] - b[4] - b[6] -
+                    b[8] - b[10];
+    return 0;
+}
+
+int rt6d_set_option(rt6d_ctx *c, int option, int value)
+{
+    if (!c) return fail(RT6D_EINVAL, "null context");
+    switch (option) {
+    case RT6D_OPT_EXACT:
+        if (value && c->prec != RT6D_FP64) return fail(RT6D_EINVAL, "exact arithmetic needs RT6D_FP64");
+        c->opt_exact = value ? 1 : 0;
+        c->fl_src = c->de_src = 0;
+        return 0;
+    case RT6D_OPT_PATH:
+        if (value < 0 || value > 1) return fail(RT6D_EINVAL, "path option %d", value);
+        c->opt_path = value;
+        c->params_ready = false;  // takes effect at the next calc_parameter
+        return 0;
+    case RT6D_OPT_BLOCK:
+        if (value < 64 || value > 1024 || (value & 63)) return fail(RT6D_EINVAL, "block %d", value);
+        c->opt_block = value;
+        return 0;
+    default:
+        return fail(RT6D_EINVAL, "unknown option %d", option);
+    }
+}
+
+int rt6d_query(rt6d_ctx *c, int what, long long *value)
+{
+    if (!c || !value) return fail(RT6D_EINVAL, "rt6d_query: null argument");
+    switch (what) {
+    case RT6D_Q_HOMOGENEOUS: *value = c->homog ? 1 : 0; return 0;
+    case RT6D_Q_DEVICE_BYTES: *value = (long long)c->dev_bytes; return 0;
+    case RT6D_Q_CELLS: *value = (long long)c->cells; return 0;
+    case RT6D_Q_STEPS: *value = c->steps; return 0;
+    case RT6D_Q_BYTES_PER_CELL: {
+        // E read + E write, plus the streamed coefficient fields on the general path
+        double b = 2.0 * c->esz;
+        if (!c->homog) {
+            for (int d = 0; d < kAxes; ++d)
+                if (c->n[d] > 1) b += (double)c->esz * (c->n[d] + 1) / c->n[d];
+            b += c->esz;  // lambda
+        }
+        *value = (long long)llround(b * 1000.0);  // milli-bytes per cell
+        return 0;
+    }
+    default: return fail(RT6D_EINVAL, "rt6d_query: unknown item %d", what);
+    }
+}
+
+int rt6d_synchronize(rt6d_ctx *c)
+{
+    int r;
+    if ((r = check_ctx(c))) return r;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+}  // extern "C"

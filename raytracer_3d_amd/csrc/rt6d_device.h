@@ -1,0 +1,141 @@
+// rt6d_device.h -- device-side building blocks of the 6D step (gfx950).
+//
+// Layout in HBM: every 6D field is stored without halos, Fortran order over
+// (x,y,z,k,l,m), linear index lin = i + nx*(j + ny*(k + ...)) with 0-based
+// coordinates.  Face fields of axis d (group velocities, fluxes) have n_d+1
+// entries along d: face f lies between cells f and f+1 (1-based), face 0 and
+// face n_d are the domain boundaries.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rt6d {
+
+constexpr int kAxes = 6;
+
+// Unsigned 32-bit division by a run-time constant: q = (umulhi(x, mul) + x) >> shift
+// (round-up multiplier method, Granlund & Montgomery 1994).  Valid for x < 2^31.
+struct FastDiv {
+    uint32_t d, mul, shift;
+};
+
+inline FastDiv make_fastdiv(uint32_t d)
+{
+    FastDiv f;
+    f.d = d;
+    uint32_t l = 0;
+    while ((1ull << l) < d) ++l;
+    f.shift = l;
+    f.mul = (uint32_t)((((1ull << 32) * ((1ull << l) - d)) / d) + 1);
+    return f;
+}
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t x, const FastDiv &f)
+{
+    return (__umulhi(x, f.mul) + x) >> f.shift;
+}
+
+// Geometry of one integration, passed by value to kernels.
+struct Geo {
+    int n[kAxes];        // interior extents
+    int inner[kAxes];    // product of the extents of the axes below d
+    FastDiv div[5];      // division by n[0..4]
+    int act;             // bit d set when n[d] > 1 (the axis is advected)
+    int up, down;        // z reflection switches
+    uint32_t ncells;
+};
+
+// Decode lin into 1-based coordinates c[d] and quotients q[d] = lin / inner[d].
+__device__ __forceinline__ void decode(const Geo &g, uint32_t lin, int c[kAxes], uint32_t q[kAxes + 1])
+{
+    q[0] = lin;
+#pragma unroll
+    for (int d = 0; d < 5; ++d) {
+        q[d + 1] = fdiv(q[d], g.div[d]);
+        c[d] = (int)(q[d] - q[d + 1] * (uint32_t)g.n[d]) + 1;
+    }
+    c[5] = (int)q[5] + 1;
+    q[6] = 0;
+}
+
+// Offset (relative to the cell) of the neighbour at coordinate c+o along axis
+// d, after the ghost treatment of integrate.f90:14-94: zero-gradient copies on
+// every open side, and on z the m-mirrored copy of reflection_condition
+// (parallel_mpi.f90:837-841,867-871) when that side reflects.
+template <int D>
+__device__ __forceinline__ int64_t nbr_off(const Geo &g, const int c[kAxes], int o)
+{
+    int cc = c[D] + o;
+    const int n = g.n[D];
+    if (D == 2) {
+        if (cc < 1) {
+            if (g.down) {
+                int zr = 1 - cc;                    // E(1-i) = E(i, nm+1-m)
+                int mr = g.n[5] + 1 - c[5];
+                return (int64_t)(zr - c[2]) * g.inner[2] + (int64_t)(mr - c[5]) * g.inner[5];
+            }
+            cc = 1;
+        } else if (cc > n) {
+            if (g.up) {
+                int zr = 2 * n + 1 - cc;            // E(nz+i) = E(nz-i+1, nm+1-m)
+                int mr = g.n[5] + 1 - c[5];
+                return (int64_t)(zr - c[2]) * g.inner[2] + (int64_t)(mr - c[5]) * g.inner[5];
+            }
+            cc = n;
+        }
+    } else {
+        cc = cc < 1 ? 1 : (cc > n ? n : cc);
+    }
+    return (int64_t)(cc - c[D]) * g.inner[D];
+}
+
+// Superbee limiter of superbee_fluxes.f90:10.
+template <typename T>
+__device__ __forceinline__ T sb_limiter(T cr)
+{
+    T a = T(2) * cr;
+    a = a < T(1) ? a : T(1);
+    T b = cr < T(2) ? cr : T(2);
+    T c = a > b ? a : b;
+    return c > T(0) ? c : T(0);
+}
+
+// Face flux in the reference's operation order (superbee_fluxes.f90:19-30):
+// IEEE divides, the single-precision 1.E20 of :26, no contraction.
+__device__ __forceinline__ double sb_flux_exact(double v, double dt, double dxf,
+                                                double Em1, double E0, double E1, double E2)
+{
+#pragma clang fp contract(off)
+    const double big = (double)1.0e20f;
+    double ucfl = fabs(v * dt / dxf);
+    double Rjp = E2 - E1;
+    double Rj = E1 - E0;
+    double Rjm = E0 - Em1;
+    double up = v > 0.0 ? Rjm : Rjp;
+    double cr;
+    if (Rj != 0.0) cr = up / Rj;
+    else cr = up * big;
+    cr = sb_limiter(cr);
+    return v * (E1 + E0) * 0.5 - fabs(v) * ((1.0 - cr) + ucfl * cr) * Rj * 0.5;
+}
+
+// The same flux without a divide: phi(r)*R written with min/max of R and the
+// upwind difference (r = Ru/R), which is phi(r)*R exactly in real arithmetic
+// and 0 for R == 0 as in the reference (where the anti-diffusive term is
+// multiplied by Rj = 0).  F = 0.5*(v*(E1+E0) - |v|*(R - (1-c)*phiR)).
+template <typename T>
+__device__ __forceinline__ T sb_flux_fast(T v, T cfl_coef, T Em1, T E0, T E1, T E2)
+{
+    T R = E1 - E0;
+    T Ru = v > T(0) ? (E0 - Em1) : (E2 - E1);
+    T aR = fabs(R);
+    T su = R < T(0) ? -Ru : Ru;
+    T p = fmax(fmin(aR, su + su), fmin(aR + aR, su));
+    p = fmax(p, T(0));
+    T phiR = R < T(0) ? -p : p;
+    T av = fabs(v);
+    T c = av * cfl_coef;
+    return T(0.5) * (v * (E1 + E0) - av * (R - (T(1) - c) * phiR));
+}
+
+}  // namespace rt6d

@@ -1,0 +1,485 @@
+// rt6d_exact.hip -- kernels that follow the reference's operation order.
+// Compiled with -ffp-contract=off: every product, sum and quotient is rounded
+// separately, as the Fortran statements are written.
+//
+//  * calc_parameter (for_src/calc_parameter.f90:2-264): dispersion relation,
+//    finite-difference group velocities / refraction rates with their
+//    boundary clipping, and the mean-flow growth rates.  Each output element
+//    is computed directly from omega_e at the halo points it depends on, so no
+//    halo'd 6D temporaries are stored.
+//  * the EXACT variants of the step, tendency and diagnose kernels.
+//  * small reductions (velocity maxima, CFL) and precision conversions.
+#include "rt6d_kernels.h"
+
+namespace rt6d {
+
+// ------------------------------------------------------- dispersion relation
+
+struct PDev {
+    ParamIn pi;
+    Geo out;       // geometry of the array being written
+};
+
+__device__ __forceinline__ int h3(const ParamIn &pi, int i, int j, int k)
+{
+    return (i + 1) + (pi.n[0] + 4) * ((j + 1) + (pi.n[1] + 4) * (k + 1));
+}
+
+// omega_i, calc_parameter.f90:19-20, at Fortran indices (may be ghosts)
+__device__ __forceinline__ double omega_i_at(const ParamIn &pi, int i, int j, int k, int l, int n, int m)
+{
+    const double kt = pi.t[3][l + 1], lt = pi.t[4][n + 1], mt = pi.t[5][m + 1];
+    const double kh2 = lt * lt + kt * kt;
+    const double mm = mt * mt;
+    const double N2 = pi.Nsqr[h3(pi, i, j, k)];
+    return pi.sign * sqrt((N2 * kh2 + (pi.f0 * pi.f0) * mm) / (mm + kh2));
+}
+
+// omega_e, calc_parameter.f90:21
+__device__ __forceinline__ double omega_e_at(const ParamIn &pi, int i, int j, int k, int l, int n, int m)
+{
+    const double oi = omega_i_at(pi, i, j, k, l, n, m);
+    const int q = h3(pi, i, j, k);
+    return oi + pi.t[3][l + 1] * pi.U[q] + pi.t[4][n + 1] * pi.V[q];
+}
+
+__device__ __forceinline__ double clip_hi(double v) { return v > 0.0 ? v : 0.0; }          // max(0d0, v)
+__device__ __forceinline__ double clip_lo(double v) { double w = -v; return -(w > 0.0 ? w : 0.0); } // -max(0d0,-v)
+
+// Analytic group velocity, calc_parameter.f90:97-98 (wave axis 3 -> kt, 4 -> lt)
+__device__ __forceinline__ double analytic_vel(const ParamIn &pi, int axis_wave, int i, int j, int k, int l, int n, int m)
+{
+    const double kt = pi.t[3][l + 1], lt = pi.t[4][n + 1], mt = pi.t[5][m + 1];
+    const double kh2 = lt * lt + kt * kt;
+    const double mm = mt * mt;
+    const int q = h3(pi, i, j, k);
+    const double oi = omega_i_at(pi, i, j, k, l, n, m);
+    const double den = mm + kh2;
+    const double base = axis_wave == 3 ? pi.U[q] : pi.V[q];
+    const double w = axis_wave == 3 ? kt : lt;
+    return base + (pi.Nsqr[q] - pi.f0 * pi.f0) / oi * mm / (den * den) * w;
+}
+
+// omega_e with coordinate `a` of axis A replaced by `v`
+__device__ __forceinline__ double oe_sub(const ParamIn &pi, const int c[kAxes], int A, int v, int B, int w)
+{
+    int cc[kAxes];
+#pragma unroll
+    for (int d = 0; d < kAxes; ++d) cc[d] = c[d];
+    cc[A] = v;
+    cc[B] = w;
+    return omega_e_at(pi, cc[0], cc[1], cc[2], cc[3], cc[4], cc[5]);
+}
+
+// Face velocity of axis D at face array coordinates c (c[D] = face index f).
+template <int D>
+__global__ void __launch_bounds__(256) k_param_velocity(const PDev P, double *__restrict__ out)
+{
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= P.out.ncells) return;
+    int c[kAxes];
+    uint32_t q[kAxes + 1];
+    decode(P.out, e, c, q);
+    c[D] -= 1;  // face index f = 0..n_D (Fortran index of the T point the value sits on)
+    const ParamIn &pi = P.pi;
+    const int *n = pi.n;
+    double v = 0.0;
+    if (D == 0 || D == 3 || D == 1 || D == 4) {
+        // x/k pair (0,3) or y/l pair (1,4)
+        constexpr int S = (D == 0 || D == 3) ? 0 : 1;   // space axis
+        constexpr int W = S + 3;                         // wavenumber axis
+        const bool fd = n[S] > 1 && n[W] > 1;
+        if (D == S) {
+            const int f = c[S];
+            if (fd) {
+                // wavenumber average then difference, :38-46 / :116-124
+                const int l = c[W];
+                const double am = 0.5 * (oe_sub(pi, c, S, f, W, l - 1) + oe_sub(pi, c, S, f, W, l));
+                const double ap = 0.5 * (oe_sub(pi, c, S, f, W, l) + oe_sub(pi, c, S, f, W, l + 1));
+                v = (ap - am) / pi.dt[W][l + 1];
+                if (f == n[S]) v = clip_hi(v);          // :56 / :134
+                else if (f == 0) v = clip_lo(v);        // :62 / :140
+            } else {
+                // analytic on the interior only; ghost face 0 keeps its zero, :89-105
+                if (f >= 1) v = analytic_vel(pi, W, c[0], c[1], c[2], c[3], c[4], c[5]);
+            }
+        } else {
+            const int f = c[W];
+            if (fd) {
+                // refraction: wavenumber-face average, space backward difference, :69-71 / :147-149
+                const int i = c[S];
+                const double a0 = 0.5 * (oe_sub(pi, c, S, i - 1, W, f) + oe_sub(pi, c, S, i - 1, W, f + 1));
+                const double a1 = 0.5 * (oe_sub(pi, c, S, i, W, f) + oe_sub(pi, c, S, i, W, f + 1));
+                v = -(a1 - a0) / pi.dt[S][i + 1];
+                if (f == n[W]) v = clip_hi(v);          // :77 / :155
+                else if (f == 0) v = clip_lo(v);        // :83 / :161
+            }
+        }
+    } else if (D == 2) {
+        // zdot = d omega / dm at z T point f, :188-216
+        const int f = c[2], m = c[5];
+        const double am = 0.5 * (oe_sub(pi, c, 2, f, 5, m - 1) + oe_sub(pi, c, 2, f, 5, m));
+        const double ap = 0.5 * (oe_sub(pi, c, 2, f, 5, m) + oe_sub(pi, c, 2, f, 5, m + 1));
+        v = (ap - am) / pi.dt[5][m + 1];
+        if (!pi.up && f == n[2]) v = clip_hi(v);
+        if (!pi.down && f == 0) v = clip_lo(v);
+    } else {
+        // mdot = -d omega / dz at m face f, :219-237
+        const int f = c[5], k = c[2];
+        const double a0 = 0.5 * (oe_sub(pi, c, 2, k - 1, 5, f) + oe_sub(pi, c, 2, k - 1, 5, f + 1));
+        const double a1 = 0.5 * (oe_sub(pi, c, 2, k, 5, f) + oe_sub(pi, c, 2, k, 5, f + 1));
+        v = -(a1 - a0) / pi.dt[2][k + 1];
+        if (f == n[5]) v = clip_hi(v);
+        else if (f == 0) v = clip_lo(v);
+    }
+    out[e] = v;
+}
+
+// lambda_x/y/z on the interior (calc_parameter.f90:240-260) and their sum
+// over the active spatial axes, in integrate.f90:204-252 order.
+__global__ void __launch_bounds__(256) k_param_lambda(const PDev P, const double *__restrict__ vx,
+                                                      const double *__restrict__ vy, const double *__restrict__ vz,
+                                                      double *__restrict__ lsum, double *__restrict__ lx,
+                                                      double *__restrict__ ly, double *__restrict__ lz)
+{
+    const uint32_t lin = blockIdx.x * blockDim.x + threadIdx.x;
+    const Geo &g = P.out;
+    if (lin >= g.ncells) return;
+    int c[kAxes];
+    uint32_t q[kAxes + 1];
+    decode(g, lin, c, q);
+    const ParamIn &pi = P.pi;
+    const int i = c[0], j = c[1], k = c[2], l = c[3], n = c[4], m = c[5];
+    // xdot(i), ydot(j), zdot(k) at the T point = face index c on the face arrays
+    const double xd = vx[lin + (uint32_t)g.inner[0] * (q[1] + 1)];
+    const double yd = vy[lin + (uint32_t)g.inner[1] * (q[2] + 1)];
+    const double zd = vz[lin + (uint32_t)g.inner[2] * (q[3] + 1)];
+    const double oi = omega_i_at(pi, i, j, k, l, n, m);
+    const double kt = pi.t[3][l + 1], lt = pi.t[4][n + 1];
+    const int q0 = h3(pi, i, j, k);
+    const double Uq = pi.U[q0], Vq = pi.V[q0];
+    const double dxt = pi.dt[0][i + 1], dyt = pi.dt[1][j + 1], dzt = pi.dt[2][k + 1];
+    const double rw = 1.0 / oi;
+    const double lamx = rw * kt *
+        ((xd - Uq) * (pi.U[h3(pi, i + 1, j, k)] - pi.U[h3(pi, i - 1, j, k)]) / (2.0 * dxt) +
+         (yd - Vq) * (pi.V[h3(pi, i + 1, j, k)] - pi.V[h3(pi, i - 1, j, k)]) / (2.0 * dxt));
+    const double lamy = rw * lt *
+        ((xd - Uq) * (pi.U[h3(pi, i, j + 1, k)] - pi.U[h3(pi, i, j - 1, k)]) / (2.0 * dyt) +
+         (yd - Vq) * (pi.V[h3(pi, i, j + 1, k)] - pi.V[h3(pi, i, j - 1, k)]) / (2.0 * dyt));
+    const double lamz = rw * zd *
+        (kt * (pi.U[h3(pi, i, j, k + 1)] - pi.U[h3(pi, i, j, k - 1)]) / (2.0 * dzt) +
+         lt * (pi.V[h3(pi, i, j, k + 1)] - pi.V[h3(pi, i, j, k - 1)]) / (2.0 * dzt));
+    double s = 0.0;
+    if (pi.n[0] > 1) s = s + lamx;
+    if (pi.n[1] > 1) s = s + lamy;
+    if (pi.n[2] > 1) s = s + lamz;
+    lsum[lin] = s;
+    if (lx) { lx[lin] = lamx; ly[lin] = lamy; lz[lin] = lamz; }
+}
+
+__global__ void __launch_bounds__(256) k_param_omega(const PDev P, double *__restrict__ oi, double *__restrict__ oe)
+{
+    const uint32_t lin = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lin >= P.out.ncells) return;
+    int c[kAxes];
+    uint32_t q[kAxes + 1];
+    decode(P.out, lin, c, q);
+    if (oi) oi[lin] = omega_i_at(P.pi, c[0], c[1], c[2], c[3], c[4], c[5]);
+    if (oe) oe[lin] = omega_e_at(P.pi, c[0], c[1], c[2], c[3], c[4], c[5]);
+}
+
+static Geo make_geo(const int *n)
+{
+    Geo g{};
+    uint64_t cells = 1;
+    for (int d = 0; d < kAxes; ++d) {
+        g.n[d] = n[d];
+        g.inner[d] = (int)cells;
+        cells *= (uint64_t)n[d];
+        if (n[d] > 1) g.act |= 1 << d;
+    }
+    for (int d = 0; d < 5; ++d) g.div[d] = make_fastdiv((uint32_t)n[d]);
+    g.ncells = (uint32_t)cells;
+    return g;
+}
+
+static inline uint32_t nblocks(uint64_t n, int b) { return (uint32_t)((n + b - 1) / b); }
+
+hipError_t launch_param_velocity(int D, const ParamIn &pi, double *out, hipStream_t s)
+{
+    int nf[kAxes];
+    for (int d = 0; d < kAxes; ++d) nf[d] = pi.n[d];
+    nf[D] += 1;
+    PDev P{pi, make_geo(nf)};
+    const uint32_t nb = nblocks(P.out.ncells, 256);
+    switch (D) {
+    case 0: hipLaunchKernelGGL(k_param_velocity<0>, dim3(nb), dim3(256), 0, s, P, out); break;
+    case 1: hipLaunchKernelGGL(k_param_velocity<1>, dim3(nb), dim3(256), 0, s, P, out); break;
+    case 2: hipLaunchKernelGGL(k_param_velocity<2>, dim3(nb), dim3(256), 0, s, P, out); break;
+    case 3: hipLaunchKernelGGL(k_param_velocity<3>, dim3(nb), dim3(256), 0, s, P, out); break;
+    case 4: hipLaunchKernelGGL(k_param_velocity<4>, dim3(nb), dim3(256), 0, s, P, out); break;
+    default: hipLaunchKernelGGL(k_param_velocity<5>, dim3(nb), dim3(256), 0, s, P, out); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_param_lambda(const ParamIn &pi, const double *const vel[kAxes], double *lam_sum,
+                               double *lam_xyz, hipStream_t s)
+{
+    PDev P{pi, make_geo(pi.n)};
+    const uint32_t nb = nblocks(P.out.ncells, 256);
+    double *lx = lam_xyz, *ly = lam_xyz ? lam_xyz + P.out.ncells : nullptr,
+           *lz = lam_xyz ? lam_xyz + 2 * (size_t)P.out.ncells : nullptr;
+    hipLaunchKernelGGL(k_param_lambda, dim3(nb), dim3(256), 0, s, P, vel[0], vel[1], vel[2], lam_sum, lx, ly, lz);
+    return hipGetLastError();
+}
+
+hipError_t launch_param_omega(const ParamIn &pi, double *omega_i, double *omega_e, hipStream_t s)
+{
+    PDev P{pi, make_geo(pi.n)};
+    hipLaunchKernelGGL(k_param_omega, dim3(nblocks(P.out.ncells, 256)), dim3(256), 0, s, P, omega_i, omega_e);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------ homogeneous tables
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_homog_tables(const Geo g, const T *__restrict__ vx, const T *__restrict__ vy,
+                                                      const T *__restrict__ vz, const T *__restrict__ vm,
+                                                      const T *__restrict__ lam, T *vx3, T *vy3, T *hz, T *hm,
+                                                      T *hlam)
+{
+    const uint32_t nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const uint32_t n4 = g.ncells / (nx * ny);
+    const uint32_t inner4m = (uint32_t)g.n[2] * g.n[3] * g.n[4];
+    const uint32_t nzf = n4 / nz * (nz + 1);     // z-face table size
+    const uint32_t nmf = inner4m * (g.n[5] + 1); // m-face table size
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n4) {
+        const uint32_t q2 = e;
+        vx3[0 * n4 + q2] = vx[(nx + 1) * ny * q2 + 0];
+        vx3[1 * n4 + q2] = vx[(nx + 1) * ny * q2 + (nx > 1 ? 1 : 0)];
+        vx3[2 * n4 + q2] = vx[(nx + 1) * ny * q2 + nx];
+        vy3[0 * n4 + q2] = vy[nx * (0 + (ny + 1) * q2)];
+        vy3[1 * n4 + q2] = vy[nx * ((ny > 1 ? 1 : 0) + (ny + 1) * q2)];
+        vy3[2 * n4 + q2] = vy[nx * (ny + (ny + 1) * q2)];
+        hlam[q2] = lam[nx * ny * q2];
+    }
+    if (e < nzf) hz[e] = vz[nx * ny * e];
+    if (e < nmf) hm[e] = vm[nx * ny * e];
+}
+
+// Every face element must equal its table entry; kdot/ldot must vanish.
+template <typename T>
+struct VelSet {
+    const T *v[kAxes];
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_homog_check(const Geo g, const VelSet<T> vel,
+                                                     const T *__restrict__ lam, const T *__restrict__ vx3,
+                                                     const T *__restrict__ vy3, const T *__restrict__ hz,
+                                                     const T *__restrict__ hm, const T *__restrict__ hlam, int *bad)
+{
+    const uint32_t nx = g.n[0], ny = g.n[1];
+    const uint32_t n4 = g.ncells / (nx * ny);
+    const uint32_t lin = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lin >= g.ncells) return;
+    int c[kAxes];
+    uint32_t q[kAxes + 1];
+    decode(g, lin, c, q);
+    const uint32_t q2 = q[2];
+    bool ok = lam[lin] == hlam[q2];
+    // each cell checks its right face, and its left face when it is face 0
+    for (int d = 0; d < kAxes; ++d) {
+        if (!(g.act & (1 << d))) continue;
+        const uint32_t fl = lin + (uint32_t)g.inner[d] * q[d + 1];
+        const uint32_t fr = fl + (uint32_t)g.inner[d];
+        const T a = vel.v[d][fr];
+        const T b = c[d] == 1 ? vel.v[d][fl] : T(0);
+        if (d == 0 || d == 1) {
+            const T *t = d == 0 ? vx3 : vy3;
+            ok = ok && a == t[(c[d] == g.n[d] ? 2u : 1u) * n4 + q2];
+            if (c[d] == 1) ok = ok && b == t[q2];
+        } else if (d == 2) {
+            const uint32_t hl = q2 + q[3];
+            ok = ok && a == hz[hl + 1];
+            if (c[d] == 1) ok = ok && b == hz[hl];
+        } else if (d == 5) {
+            const uint32_t inner4m = (uint32_t)g.n[2] * g.n[3] * g.n[4];
+            ok = ok && a == hm[q2 + inner4m];
+            if (c[d] == 1) ok = ok && b == hm[q2];
+        } else {
+            ok = ok && a == T(0) && b == T(0);
+        }
+    }
+    if (!ok) atomicOr(bad, 1);
+}
+
+template <typename T>
+hipError_t launch_build_homog(const Geo &g, const T *const vel[kAxes], const T *lam, T *vx3, T *vy3, T *vz, T *vm,
+                              T *hlam, int *bad_dev, hipStream_t s)
+{
+    const uint32_t n4 = g.ncells / ((uint32_t)g.n[0] * g.n[1]);
+    const uint32_t nzf = n4 / g.n[2] * (g.n[2] + 1);
+    const uint32_t nmf = (uint32_t)g.n[2] * g.n[3] * g.n[4] * (g.n[5] + 1);
+    uint32_t m = n4 > nzf ? n4 : nzf;
+    m = m > nmf ? m : nmf;
+    hipLaunchKernelGGL(k_homog_tables<T>, dim3(nblocks(m, 256)), dim3(256), 0, s, g, vel[0], vel[1], vel[2], vel[5],
+                       lam, vx3, vy3, vz, vm, hlam);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    VelSet<T> vs;
+    for (int d = 0; d < kAxes; ++d) vs.v[d] = vel[d];
+    hipLaunchKernelGGL(k_homog_check<T>, dim3(nblocks(g.ncells, 256)), dim3(256), 0, s, g, vs, lam, vx3, vy3, vz,
+                       vm, hlam, bad_dev);
+    return hipGetLastError();
+}
+template hipError_t launch_build_homog<double>(const Geo &, const double *const[kAxes], const double *, double *,
+                                               double *, double *, double *, double *, int *, hipStream_t);
+template hipError_t launch_build_homog<float>(const Geo &, const float *const[kAxes], const float *, float *, float *,
+                                              float *, float *, float *, int *, hipStream_t);
+
+// ------------------------------------------------------------ reductions
+
+__device__ __forceinline__ void atomic_max_pos(unsigned long long *bits, double v)
+{
+    // non-negative binary64 values order like their bit patterns
+    atomicMax(bits, (unsigned long long)__double_as_longlong(v));
+}
+
+__global__ void __launch_bounds__(256) k_vmax(const Geo g, const double *__restrict__ v0, const double *__restrict__ v1,
+                                              const double *__restrict__ v2, const double *__restrict__ v3,
+                                              const double *__restrict__ v4, const double *__restrict__ v5,
+                                              unsigned long long *bits)
+{
+    const uint32_t lin = blockIdx.x * blockDim.x + threadIdx.x;
+    double m[kAxes] = {0, 0, 0, 0, 0, 0};
+    if (lin < g.ncells) {
+        int c[kAxes];
+        uint32_t q[kAxes + 1];
+        decode(g, lin, c, q);
+        const double *vv[kAxes] = {v0, v1, v2, v3, v4, v5};
+#pragma unroll
+        for (int d = 0; d < kAxes; ++d) {
+            // T point c of the reference = right face of cell c
+            const uint32_t fr = lin + (uint32_t)g.inner[d] * (q[d + 1] + 1);
+            m[d] = fabs(vv[d][fr]);
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < kAxes; ++d) {
+        const double w = wave_max(m[d]);
+        if ((threadIdx.x & 63) == 0) atomic_max_pos(&bits[d], w);
+    }
+}
+
+hipError_t launch_vmax(const Geo &g, const double *const vel[kAxes], unsigned long long *vmax_bits, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_vmax, dim3(nblocks(g.ncells, 256)), dim3(256), 0, s, g, vel[0], vel[1], vel[2], vel[3],
+                       vel[4], vel[5], vmax_bits);
+    return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) k_cfl_max(const Geo g, const double *__restrict__ v0, const double *__restrict__ v1,
+                                                 const double *__restrict__ v2, const double *__restrict__ v3,
+                                                 const double *__restrict__ v4, const double *__restrict__ v5,
+                                                 const AxisCoef ax, unsigned long long *bits)
+{
+    const uint32_t lin = blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0.0;
+    if (lin < g.ncells) {
+        int c[kAxes];
+        uint32_t q[kAxes + 1];
+        decode(g, lin, c, q);
+        const double *vv[kAxes] = {v0, v1, v2, v3, v4, v5};
+#pragma unroll
+        for (int d = 0; d < kAxes; ++d) {
+            if (!(g.act & (1 << d))) continue;
+            const uint32_t fl = lin + (uint32_t)g.inner[d] * q[d + 1];
+            const double a = fabs(vv[d][fl]), b = fabs(vv[d][fl + (uint32_t)g.inner[d]]);
+            s += (a > b ? a : b) / ax.dx[d][c[d]];
+        }
+    }
+    const double w = wave_max(s);
+    if ((threadIdx.x & 63) == 0) atomic_max_pos(bits, w);
+}
+
+hipError_t launch_cfl_max(const Geo &g, const double *const vel[kAxes], const AxisCoef &ax, unsigned long long *bits,
+                          hipStream_t s)
+{
+    hipLaunchKernelGGL(k_cfl_max, dim3(nblocks(g.ncells, 256)), dim3(256), 0, s, g, vel[0], vel[1], vel[2], vel[3],
+                       vel[4], vel[5], ax, bits);
+    return hipGetLastError();
+}
+
+__global__ void k_d2f(const double *__restrict__ in, float *__restrict__ out, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = (float)in[i];
+}
+__global__ void k_f2d(const float *__restrict__ in, double *__restrict__ out, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = (double)in[i];
+}
+hipError_t launch_convert_d2f(const double *in, float *out, size_t n, hipStream_t s)
+{
+    uint32_t nb = nblocks(n, 256);
+    nb = nb > 65536 ? 65536 : (nb ? nb : 1);
+    hipLaunchKernelGGL(k_d2f, dim3(nb), dim3(256), 0, s, in, out, n);
+    return hipGetLastError();
+}
+hipError_t launch_convert_f2d(const float *in, double *out, size_t n, hipStream_t s)
+{
+    uint32_t nb = nblocks(n, 256);
+    nb = nb > 65536 ? 65536 : (nb ? nb : 1);
+    hipLaunchKernelGGL(k_f2d, dim3(nb), dim3(256), 0, s, in, out, n);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ EXACT step
+
+hipError_t launch_step_exact(const Geo &g, const StepPtrs<double> &p, bool homog, const LaunchCfg &lc)
+{
+    const uint32_t nbl = nblocks(g.ncells, lc.block);
+    const uint32_t nbh = ((nbl + 7u) / 8u) * 8u;
+    if (homog)
+        hipLaunchKernelGGL((k_step<double, true, true>), dim3(nbh), dim3(lc.block), 0, lc.stream, g, p, nbl);
+    else
+        hipLaunchKernelGGL((k_step<double, true, false>), dim3(nbh), dim3(lc.block), 0, lc.stream, g, p, nbl);
+    return hipGetLastError();
+}
+
+hipError_t launch_tendency_exact(const Geo &g, const StepPtrs<double> &p, bool homog, hipStream_t s)
+{
+    const uint32_t nb = nblocks(g.ncells, 256);
+    if (homog) hipLaunchKernelGGL((k_tendency<double, true, true>), dim3(nb), dim3(256), 0, s, g, p);
+    else hipLaunchKernelGGL((k_tendency<double, true, false>), dim3(nb), dim3(256), 0, s, g, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_diag_exact(const Geo &g, const StepPtrs<double> &p, bool homog, const double *Ecur,
+                             double *partial, uint32_t nparts, double *out, double *Eint, double *mfint,
+                             double *diss, hipStream_t s)
+{
+    const uint32_t nb = nblocks(g.ncells, 256);
+    if (nb > nparts) return hipErrorInvalidValue;
+    if (homog)
+        hipLaunchKernelGGL((k_diag_cells<double, true, true>), dim3(nb), dim3(256), 0, s, g, p, Ecur, partial);
+    else
+        hipLaunchKernelGGL((k_diag_cells<double, true, false>), dim3(nb), dim3(256), 0, s, g, p, Ecur, partial);
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(1024), 0, s, partial, nb, out);
+    if (Eint || mfint || diss) {
+        const uint32_t n3 = (uint32_t)g.n[0] * g.n[1] * g.n[2];
+        if (homog)
+            hipLaunchKernelGGL((k_diag_columns<double, true, true>), dim3(nblocks(n3, 256)), dim3(256), 0, s, g, p,
+                               Ecur, Eint, mfint, diss);
+        else
+            hipLaunchKernelGGL((k_diag_columns<double, true, false>), dim3(nblocks(n3, 256)), dim3(256), 0, s, g, p,
+                               Ecur, Eint, mfint, diss);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace rt6d

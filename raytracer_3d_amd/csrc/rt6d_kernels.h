@@ -1,0 +1,434 @@
+// rt6d_kernels.h -- kernel templates of the 6D step, instantiated in
+// rt6d_exact.hip (compiled with -ffp-contract=off: reference operation order)
+// and rt6d_fast.hip (reciprocals, division-free limiter, FMA contraction).
+#pragma once
+#include "rt6d_device.h"
+
+namespace rt6d {
+
+// Per-axis 1D coefficient block in device memory, n_d+1 entries each:
+//   dx[f]   = dxt_d(f), f = 0..n_d (f = 0 is the ghost spacing)
+//   rdx[f]  = 1 / dx[f]
+//   cfl[f]  = dt / dx[f]  (recomputed when dt changes)
+struct AxisCoef {
+    const double *dx[kAxes];
+    const double *rdx[kAxes];
+    const double *cfl[kAxes];
+};
+
+// Horizontally homogeneous coefficient tables over the 4D (z,k,l,m) space
+// (q2 = lin / (nx*ny)):
+//   vx3[cls*n4 + q2], vy3 likewise: cls 0 face 0, 1 interior faces, 2 face n
+//   vz[(q2 + q3)]  faces along z: extent nz+1
+//   vm[q2 + f*inner4m] faces along m: extent nm+1
+//   lam[q2]
+template <typename T>
+struct HomTables {
+    const T *vx3, *vy3, *vz, *vm, *lam;
+    uint32_t n4, inner4m;
+};
+
+template <typename T>
+struct StepPtrs {
+    const T *Ein;
+    T *Eout;
+    const T *vel[kAxes];   // general path: face arrays of each axis
+    const T *lam;          // general path: lambda summed over active x,y,z
+    HomTables<T> hom;
+    AxisCoef ax;
+    double dt;
+    // tendency outputs (dE, meanfl, face fluxes), used by the tendency kernel
+    T *dE, *meanfl;
+    T *flux[kAxes];
+};
+
+// Map a hardware block id to a logical block so that the 8 XCDs (hardware
+// round-robins blocks over them) each work on one contiguous eighth of the
+// cell range: neighbours along the strided axes then share an XCD's L2.
+__device__ __forceinline__ uint32_t xcd_logical_block(uint32_t b, uint32_t nb_logical)
+{
+    const uint32_t per = (nb_logical + 7u) / 8u;
+    return (b & 7u) * per + (b >> 3);
+}
+
+template <int D, typename T, bool HOMOG>
+__device__ __forceinline__ void load_vel(const Geo &g, const StepPtrs<T> &p, uint32_t lin,
+                                         const int c[kAxes], const uint32_t q[kAxes + 1], T &vL, T &vR)
+{
+    if (!HOMOG) {
+        const uint32_t fl = lin + (uint32_t)g.inner[D] * q[D + 1];
+        vL = p.vel[D][fl];
+        vR = p.vel[D][fl + (uint32_t)g.inner[D]];
+    } else {
+        const uint32_t q2 = q[2];
+        if (D == 0 || D == 1) {
+            const T *t = D == 0 ? p.hom.vx3 : p.hom.vy3;
+            const int cd = c[D], nd = g.n[D];
+            vL = t[(cd == 1 ? 0u : 1u) * p.hom.n4 + q2];
+            vR = t[(cd == nd ? 2u : 1u) * p.hom.n4 + q2];
+        } else if (D == 2) {
+            const uint32_t fl = q2 + q[3];
+            vL = p.hom.vz[fl];
+            vR = p.hom.vz[fl + 1];
+        } else {  // D == 5; k and l have no HOMOG velocities (identically zero)
+            vL = p.hom.vm[q2];
+            vR = p.hom.vm[q2 + p.hom.inner4m];
+        }
+    }
+}
+
+// Contribution of axis D to the tendency of one cell: loads the four
+// neighbours, computes the two face fluxes and subtracts their divergence
+// from dE (integrate.f90:105-200 order).  FL/FR receive the face fluxes.
+template <int D, typename T, bool EXACT, bool HOMOG>
+__device__ __forceinline__ void axis_term(const Geo &g, const StepPtrs<T> &p, const T *__restrict__ E,
+                                          uint32_t lin, const int c[kAxes], const uint32_t q[kAxes + 1],
+                                          T e0, T &dE, T &FL, T &FR)
+{
+    const T em2 = E[(int64_t)lin + nbr_off<D>(g, c, -2)];
+    const T em1 = E[(int64_t)lin + nbr_off<D>(g, c, -1)];
+    const T ep1 = E[(int64_t)lin + nbr_off<D>(g, c, 1)];
+    const T ep2 = E[(int64_t)lin + nbr_off<D>(g, c, 2)];
+    T vL, vR;
+    load_vel<D, T, HOMOG>(g, p, lin, c, q, vL, vR);
+    const int cd = c[D];
+    if constexpr (EXACT) {
+        const double *dx = p.ax.dx[D];
+        FL = sb_flux_exact(vL, p.dt, dx[cd - 1], em2, em1, e0, ep1);
+        FR = sb_flux_exact(vR, p.dt, dx[cd], em1, e0, ep1, ep2);
+        dE = dE - (FR - FL) / dx[cd];
+    } else {
+        const double *cf = p.ax.cfl[D];
+        FL = sb_flux_fast<T>(vL, (T)cf[cd - 1], em2, em1, e0, ep1);
+        FR = sb_flux_fast<T>(vR, (T)cf[cd], em1, e0, ep1, ep2);
+        dE = dE - (FR - FL) * (T)p.ax.rdx[D][cd];
+    }
+}
+
+// Whole tendency of one cell: dE (divergence minus meanfl) and meanfl.
+// fl/fr receive the face fluxes of each axis (zero when the axis is skipped).
+template <typename T, bool EXACT, bool HOMOG>
+__device__ __forceinline__ void cell_tendency(const Geo &g, const StepPtrs<T> &p, const T *__restrict__ E,
+                                              uint32_t lin, const int c[kAxes], const uint32_t q[kAxes + 1],
+                                              T e0, T &dE, T &mf, T fl[kAxes], T fr[kAxes])
+{
+    dE = T(0);
+#pragma unroll
+    for (int d = 0; d < kAxes; ++d) { fl[d] = T(0); fr[d] = T(0); }
+    if (g.act & 1) axis_term<0, T, EXACT, HOMOG>(g, p, E, lin, c, q, e0, dE, fl[0], fr[0]);
+    if (g.act & 2) axis_term<1, T, EXACT, HOMOG>(g, p, E, lin, c, q, e0, dE, fl[1], fr[1]);
+    if (g.act & 4) axis_term<2, T, EXACT, HOMOG>(g, p, E, lin, c, q, e0, dE, fl[2], fr[2]);
+    if (!HOMOG) {
+        if (g.act & 8) axis_term<3, T, EXACT, HOMOG>(g, p, E, lin, c, q, e0, dE, fl[3], fr[3]);
+        if (g.act & 16) axis_term<4, T, EXACT, HOMOG>(g, p, E, lin, c, q, e0, dE, fl[4], fr[4]);
+    }
+    if (g.act & 32) axis_term<5, T, EXACT, HOMOG>(g, p, E, lin, c, q, e0, dE, fl[5], fr[5]);
+    const T lam = HOMOG ? p.hom.lam[q[2]] : p.lam[lin];
+    mf = e0 * lam;            // meanfl = E*lambda, integrate.f90:202-252
+    dE = dE - mf;             // integrate.f90:261
+}
+
+// One forward-Euler step (integrate.f90:4-271) for the cells of one block.
+template <typename T, bool EXACT, bool HOMOG>
+__global__ void __launch_bounds__(1024) k_step(const Geo g, const StepPtrs<T> p, uint32_t nb_logical)
+{
+    const uint32_t b = xcd_logical_block(blockIdx.x, nb_logical);
+    if (b >= nb_logical) return;
+    const uint32_t lin = b * blockDim.x + threadIdx.x;
+    if (lin >= g.ncells) return;
+    int c[kAxes];
+    uint32_t q[kAxes + 1];
+    decode(g, lin, c, q);
+    const T *__restrict__ E = p.Ein;
+    const T e0 = E[lin];
+    T dE, mf, fl[kAxes], fr[kAxes];
+    cell_tendency<T, EXACT, HOMOG>(g, p, E, lin, c, q, e0, dE, mf, fl, fr);
+    if constexpr (EXACT) {
+        p.Eout[lin] = e0 + p.dt * dE;    // integrate.f90:262, separately rounded
+    } else {
+        p.Eout[lin] = e0 + (T)p.dt * dE;
+    }
+}
+
+// Tendency fields of one step (the by-products integrate leaves behind:
+// dE, meanfl, flux_x..flux_m), computed from the state the step started from.
+template <typename T, bool EXACT, bool HOMOG>
+__global__ void __launch_bounds__(256) k_tendency(const Geo g, const StepPtrs<T> p)
+{
+    const uint32_t lin = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lin >= g.ncells) return;
+    int c[kAxes];
+    uint32_t q[kAxes + 1];
+    decode(g, lin, c, q);
+    const T *__restrict__ E = p.Ein;
+    const T e0 = E[lin];
+    T dE, mf, fl[kAxes], fr[kAxes];
+    cell_tendency<T, EXACT, HOMOG>(g, p, E, lin, c, q, e0, dE, mf, fl, fr);
+    p.dE[lin] = dE;
+    p.meanfl[lin] = mf;
+#pragma unroll
+    for (int d = 0; d < kAxes; ++d) {
+        if (!(g.act & (1 << d))) continue;
+        const uint32_t f = lin + (uint32_t)g.inner[d] * q[d + 1];
+        if (c[d] == 1) p.flux[d][f] = fl[d];
+        p.flux[d][f + (uint32_t)g.inner[d]] = fr[d];
+    }
+}
+
+// ---------------------------------------------------------------- diagnose
+
+// Per-block partial sums of diagnose.f90:45-68 and :102-277.
+// slot 0 eint, 1 mflint, 2 mfl_max, 3..14 bfl_xs..bfl_me
+constexpr int kDiagSlots = 15;
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+template <typename T, bool EXACT, bool HOMOG>
+__global__ void __launch_bounds__(256) k_diag_cells(const Geo g, const StepPtrs<T> p, const T *__restrict__ Ecur,
+                                                    double *__restrict__ partial)
+{
+    __shared__ double red[4][kDiagSlots];
+    const uint32_t lin = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[kDiagSlots];
+#pragma unroll
+    for (int s = 0; s < kDiagSlots; ++s) acc[s] = 0.0;
+    if (lin < g.ncells) {
+        int c[kAxes];
+        uint32_t q[kAxes + 1];
+        decode(g, lin, c, q);
+        const T *__restrict__ E = p.Ein;
+        const T e0 = E[lin];
+        T dE, mf, fl[kAxes], fr[kAxes];
+        cell_tendency<T, EXACT, HOMOG>(g, p, E, lin, c, q, e0, dE, mf, fl, fr);
+        double dxs[kAxes];
+#pragma unroll
+        for (int d = 0; d < kAxes; ++d) dxs[d] = p.ax.dx[d][c[d]];
+        const double vol = dxs[0] * dxs[1] * dxs[2] * dxs[3] * dxs[4] * dxs[5];
+        acc[0] = (double)dE * vol;
+        acc[1] = (double)mf * vol;
+        const double ec = (double)Ecur[lin];
+        const double eps = (double)1e-12f;
+        acc[2] = fabs((double)mf) / (ec > eps ? ec : eps);
+#pragma unroll
+        for (int d = 0; d < kAxes; ++d) {
+            // product of the other five spacings in axis order
+            double w = 1.0;
+            bool first = true;
+#pragma unroll
+            for (int e = 0; e < kAxes; ++e) {
+                if (e == d) continue;
+                w = first ? dxs[e] : w * dxs[e];
+                first = false;
+            }
+            if (c[d] == 1) acc[3 + 2 * d] = (double)fl[d] * w;
+            if (c[d] == g.n[d]) acc[4 + 2 * d] = (double)fr[d] * w;
+        }
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int s = 0; s < kDiagSlots; ++s) {
+        double v = s == 2 ? wave_max(acc[s]) : wave_sum(acc[s]);
+        if (lane == 0) red[wv][s] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kDiagSlots) {
+        const int s = threadIdx.x;
+        const int nw = (blockDim.x + 63) >> 6;
+        double v = red[0][s];
+        for (int w = 1; w < nw; ++w) v = s == 2 ? fmax(v, red[w][s]) : v + red[w][s];
+        partial[(size_t)blockIdx.x * kDiagSlots + s] = v;
+    }
+}
+
+// Compensated (Neumaier) reduction of the per-block partials, one block.
+static __global__ void __launch_bounds__(1024) k_diag_final(const double *__restrict__ partial, uint32_t nparts,
+                                                     double *__restrict__ out)
+{
+    __shared__ double ssum[1024], scmp[1024];
+    for (int s = 0; s < kDiagSlots; ++s) {
+        double sum = 0.0, cmp = 0.0;
+        for (uint32_t b = threadIdx.x; b < nparts; b += blockDim.x) {
+            const double x = partial[(size_t)b * kDiagSlots + s];
+            if (s == 2) { sum = fmax(sum, x); continue; }
+            const double t = sum + x;
+            cmp += fabs(sum) >= fabs(x) ? (sum - t) + x : (x - t) + sum;
+            sum = t;
+        }
+        ssum[threadIdx.x] = sum;
+        scmp[threadIdx.x] = cmp;
+        __syncthreads();
+        for (int w = blockDim.x >> 1; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) {
+                const double a = ssum[threadIdx.x], x = ssum[threadIdx.x + w];
+                if (s == 2) {
+                    ssum[threadIdx.x] = fmax(a, x);
+                } else {
+                    const double t = a + x;
+                    const double e = fabs(a) >= fabs(x) ? (a - t) + x : (x - t) + a;
+                    ssum[threadIdx.x] = t;
+                    scmp[threadIdx.x] += scmp[threadIdx.x + w] + e;
+                }
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[s] = s == 2 ? ssum[0] : ssum[0] + scmp[0];
+        __syncthreads();
+    }
+}
+
+// Wavenumber-space integrals per (x,y,z) column (diagnose.f90:84-98) and the
+// k/l/m boundary-flux columns diss_* (:186-277), summed sequentially in the
+// reference's loop order by one thread per column.
+template <typename T, bool EXACT, bool HOMOG>
+__global__ void __launch_bounds__(256) k_diag_columns(const Geo g, const StepPtrs<T> p, const T *__restrict__ Ecur,
+                                                      double *__restrict__ Eint, double *__restrict__ mfint,
+                                                      double *__restrict__ diss)
+{
+    const uint32_t n3 = (uint32_t)g.n[0] * g.n[1] * g.n[2];
+    const uint32_t col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= n3) return;
+    const int nk = g.n[3], nl = g.n[4], nm = g.n[5];
+    const uint32_t s3 = (uint32_t)g.inner[3], s4 = (uint32_t)g.inner[4], s5 = (uint32_t)g.inner[5];
+    const double *dkt = p.ax.dx[3], *dlt = p.ax.dx[4], *dmt = p.ax.dx[5];
+    double ei = 0.0, mi = 0.0;
+    for (int m = 1; m <= nm; ++m)
+        for (int n = 1; n <= nl; ++n)
+            for (int l = 1; l <= nk; ++l) {
+                const uint32_t lin = col + s3 * (l - 1) + s4 * (n - 1) + s5 * (m - 1);
+                ei = ei + (double)Ecur[lin] * dlt[n] * dmt[m] * dkt[l];
+                int c[kAxes];
+                uint32_t q[kAxes + 1];
+                decode(g, lin, c, q);
+                const T lam = HOMOG ? p.hom.lam[q[2]] : p.lam[lin];
+                const double mf = (double)(p.Ein[lin] * lam);
+                mi = mi + mf * dkt[l] * dlt[n] * dmt[m];
+            }
+    if (Eint) Eint[col] = ei;
+    if (mfint) mfint[col] = mi;
+    if (!diss) return;
+    // boundary fluxes of k, l, m at faces 0 and n_d
+    double dks = 0, dke = 0, dls = 0, dle = 0, dms = 0, dme = 0;
+    const T *__restrict__ E = p.Ein;
+    if (!HOMOG && (g.act & 8)) {
+        for (int m = 1; m <= nm; ++m)
+            for (int n = 1; n <= nl; ++n) {
+                const uint32_t l0 = col + s4 * (n - 1) + s5 * (m - 1);
+                const uint32_t l1 = l0 + s3 * (nk - 1);
+                int c[kAxes];
+                uint32_t q[kAxes + 1];
+                T dE = T(0), fl, fr;
+                decode(g, l0, c, q);
+                axis_term<3, T, EXACT, HOMOG>(g, p, E, l0, c, q, E[l0], dE, fl, fr);
+                dks = dks + (double)fl * dlt[n] * dmt[m];
+                decode(g, l1, c, q);
+                axis_term<3, T, EXACT, HOMOG>(g, p, E, l1, c, q, E[l1], dE, fl, fr);
+                dke = dke + (double)fr * dlt[n] * dmt[m];
+            }
+    }
+    if (!HOMOG && (g.act & 16)) {
+        for (int m = 1; m <= nm; ++m)
+            for (int l = 1; l <= nk; ++l) {
+                const uint32_t l0 = col + s3 * (l - 1) + s5 * (m - 1);
+                const uint32_t l1 = l0 + s4 * (nl - 1);
+                int c[kAxes];
+                uint32_t q[kAxes + 1];
+                T dE = T(0), fl, fr;
+                decode(g, l0, c, q);
+                axis_term<4, T, EXACT, HOMOG>(g, p, E, l0, c, q, E[l0], dE, fl, fr);
+                dls = dls + (double)fl * dkt[l] * dmt[m];
+                decode(g, l1, c, q);
+                axis_term<4, T, EXACT, HOMOG>(g, p, E, l1, c, q, E[l1], dE, fl, fr);
+                dle = dle + (double)fr * dkt[l] * dmt[m];
+            }
+    }
+    if (g.act & 32) {
+        for (int n = 1; n <= nl; ++n)
+            for (int l = 1; l <= nk; ++l) {
+                const uint32_t l0 = col + s3 * (l - 1) + s4 * (n - 1);
+                const uint32_t l1 = l0 + s5 * (nm - 1);
+                int c[kAxes];
+                uint32_t q[kAxes + 1];
+                T dE = T(0), fl, fr;
+                decode(g, l0, c, q);
+                axis_term<5, T, EXACT, HOMOG>(g, p, E, l0, c, q, E[l0], dE, fl, fr);
+                dms = dms + (double)fl * dkt[l] * dlt[n];
+                decode(g, l1, c, q);
+                axis_term<5, T, EXACT, HOMOG>(g, p, E, l1, c, q, E[l1], dE, fl, fr);
+                dme = dme + (double)fr * dkt[l] * dlt[n];
+            }
+    }
+    diss[0 * n3 + col] = dks;
+    diss[1 * n3 + col] = dke;
+    diss[2 * n3 + col] = dls;
+    diss[3 * n3 + col] = dle;
+    diss[4 * n3 + col] = dms;
+    diss[5 * n3 + col] = dme;
+}
+
+// ---------------------------------------------------------------- launchers
+
+struct LaunchCfg {
+    int block;
+    hipStream_t stream;
+};
+
+// Implemented in rt6d_exact.hip (EXACT, double only) and rt6d_fast.hip.
+hipError_t launch_step_exact(const Geo &g, const StepPtrs<double> &p, bool homog, const LaunchCfg &lc);
+hipError_t launch_tendency_exact(const Geo &g, const StepPtrs<double> &p, bool homog, hipStream_t s);
+hipError_t launch_diag_exact(const Geo &g, const StepPtrs<double> &p, bool homog, const double *Ecur,
+                             double *partial, uint32_t nparts, double *out, double *Eint, double *mfint,
+                             double *diss, hipStream_t s);
+template <typename T>
+hipError_t launch_step_fast(const Geo &g, const StepPtrs<T> &p, bool homog, const LaunchCfg &lc);
+template <typename T>
+hipError_t launch_tendency_fast(const Geo &g, const StepPtrs<T> &p, bool homog, hipStream_t s);
+template <typename T>
+hipError_t launch_diag_fast(const Geo &g, const StepPtrs<T> &p, bool homog, const T *Ecur, double *partial,
+                            uint32_t nparts, double *out, double *Eint, double *mfint, double *diss,
+                            hipStream_t s);
+
+// ------------------------------------------------ calc_parameter (exact TU)
+
+struct ParamIn {
+    // halo'd 1D grids (n+4), index f+1 for Fortran index f in [-1, n+2]
+    const double *t[kAxes];
+    const double *dt[kAxes];
+    const double *Nsqr, *U, *V;   // halo'd 3D
+    double f0, sign;
+    int n[kAxes];
+    int up, down;
+};
+
+// Face velocity of axis D on its face array; lambda sum; omegas.  Outputs in
+// double; the API converts to the step precision.
+hipError_t launch_param_velocity(int D, const ParamIn &pi, double *out, hipStream_t s);
+hipError_t launch_param_lambda(const ParamIn &pi, const double *const vel[kAxes], double *lam_sum,
+                               double *lam_xyz /* may be null: 3 arrays */, hipStream_t s);
+hipError_t launch_param_omega(const ParamIn &pi, double *omega_i, double *omega_e, hipStream_t s);
+// Homogeneous tables from the general fields and an exact check that every
+// cell equals its table entry (and that kdot, ldot vanish).  *ok_dev is an
+// int on the device set to 1 on any mismatch.
+template <typename T>
+hipError_t launch_build_homog(const Geo &g, const T *const vel[kAxes], const T *lam, T *vx3, T *vy3, T *vz,
+                              T *vm, T *hlam, int *bad_dev, hipStream_t s);
+// max over faces 1..n_d of |v_d| into vmax_bits[6] (as ordered 64-bit ints)
+hipError_t launch_vmax(const Geo &g, const double *const vel[kAxes], unsigned long long *vmax_bits,
+                       hipStream_t s);
+// max over cells of sum_d max(|vL|,|vR|)/dx_d(c) into *bits
+hipError_t launch_cfl_max(const Geo &g, const double *const vel[kAxes], const AxisCoef &ax,
+                          unsigned long long *bits, hipStream_t s);
+hipError_t launch_convert_d2f(const double *in, float *out, size_t n, hipStream_t s);
+hipError_t launch_convert_f2d(const float *in, double *out, size_t n, hipStream_t s);
+
+}  // namespace rt6d
