@@ -53,21 +53,23 @@ def _time_steps(m, dt, steps, warmup):
 
 def _pe_grid(n, npes):
     """A legal PE grid for the reference decomposition (parallel_mpi.f90:14-28):
-    factors of 2 over x, y (any split) and k, l (even splits only)."""
+    factors of 2 dealt round-robin over x, y (any split) and k, l (even
+    splits only), so the blocks stay as cubic as the factors allow."""
     grid = [1] * 6
     order = [0, 1, 3, 4, 2, 5]
     left = npes
-    while left > 1 and left % 2 == 0:
-        placed = False
-        for ax in order:
-            nb = grid[ax] * 2
-            if n[ax] % nb == 0 and n[ax] // nb >= 2:
-                grid[ax] = nb
-                left //= 2
-                placed = True
-                break
-        if not placed:
-            break
+    stuck = 0
+    i = 0
+    while left > 1 and left % 2 == 0 and stuck < len(order):
+        ax = order[i % len(order)]
+        i += 1
+        nb = grid[ax] * 2
+        if n[ax] % nb == 0 and n[ax] // nb >= 2:
+            grid[ax] = nb
+            left //= 2
+            stuck = 0
+        else:
+            stuck += 1
     return grid if left == 1 else None
 
 
@@ -149,8 +151,8 @@ def main():
     cells = case.cells
     workload = "gm_24x24x24_16x16x16"
 
-    def run(precision, path):
-        m = rt6d.Model.from_case(case, precision=precision, path=path)
+    def run(precision, path, kernel=0):
+        m = rt6d.Model.from_case(case, precision=precision, path=path, kernel=kernel)
         m.calc_parameter()
         dt = m.cfl_dt(0.5)
         wall, ev_ms = _time_steps(m, dt, args.steps, args.warmup)
@@ -176,8 +178,12 @@ def main():
         extras["fp32"] = run(rt6d.FP32, 0)
         extras["fp32"]["speedup_vs_fp64"] = extras["fp32"]["value"] / main_res["value"]
         extras["fp32_general"] = run(rt6d.FP32, 1)
+        _log("binary64 step, per-cell kernel")
+        extras["fp64_percell"] = run(rt6d.FP64, 0, kernel=1)
+        extras["fp64_general_percell"] = run(rt6d.FP64, 1, kernel=1)
 
-    kname = "k_step<double, false, true>" if main_res["homogeneous"] else "k_step<double, false, false>"
+    kname = ("k_step_march<double, false, true>" if main_res["homogeneous"]
+             else "k_step_march<double, false, false>")
     achieved = main_res["achieved_GBs"]
     roofline = {
         "bound": "hbm",

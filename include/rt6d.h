@@ -62,7 +62,11 @@ extern "C" {
                                 0 (default): reciprocals, division-free limiter, FMA */
 #define RT6D_OPT_PATH 2      /* 0 auto (default), 1 force the general path
                                 (6D coefficient fields streamed from HBM) */
-#define RT6D_OPT_BLOCK 3     /* threads per block of the step kernel (64..1024) */
+#define RT6D_OPT_BLOCK 3     /* threads per block of the per-cell step kernel (64..1024) */
+#define RT6D_OPT_KERNEL 4    /* step kernel: 0 auto (default, the column kernel with the most
+                                z cells per thread that divide nz); 1 per-cell kernel;
+                                2/3 LDS marching kernel, 1/2 z cells per thread (nx*ny <= 1024);
+                                4/5/6 column kernel, 1/2/4 z cells per thread */
 
 /* queries for rt6d_query */
 #define RT6D_Q_HOMOGENEOUS 1 /* 1 if the horizontally homogeneous path is active */
@@ -146,7 +150,8 @@ int rt6d_fluxes(rt6d_ctx *ctx, double dt);
  * the last integrate (as the reference leaves them after integrate), or of
  * the last rt6d_fluxes call if it came later.  Velocities/lambdas/omegas:
  * calc_parameter outputs.  layout INTERIOR, FACES (velocities, fluxes) or
- * HALO (E only; ghosts set as integrate sets them). */
+ * HALO (E only: the interior entries are written, ghost entries of the host
+ * array are left as they were). */
 int rt6d_download(rt6d_ctx *ctx, int field, int layout, double *out);
 
 /*
@@ -157,6 +162,12 @@ int rt6d_download(rt6d_ctx *ctx, int field, int layout, double *out);
  */
 int rt6d_diagnose(rt6d_ctx *ctx, double dt, rt6d_diag *out,
                   double *E_int, double *meanfl_int, double *diss6);
+
+/* Measurement helper with no reference counterpart: copies `bytes` between
+ * two scratch device buffers with a plain streaming kernel `reps` times and
+ * returns the average kernel time (ms).  Gives the achievable copy bandwidth
+ * and the reference point for the HBM byte counters of rocprofv3. */
+int rt6d_copy_bandwidth(long long bytes, int reps, double *ms);
 
 int rt6d_set_option(rt6d_ctx *ctx, int option, int value);
 int rt6d_query(rt6d_ctx *ctx, int what, long long *value);

@@ -184,7 +184,14 @@ def run_reference(case_file: str, out_file: str = "-", np_grid=None, mpirun: str
     else:
         npes = int(np.prod(np_grid))
         cmd = [mpirun, "-np", str(npes), REF_BIN, case_file, out_file] + [str(v) for v in np_grid]
-    r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, env=env)
+    def _stack_unlimited():
+        # the reference keeps 6D automatic arrays (calc_parameter.f90:7-8) on the stack
+        import resource
+        soft, hard = resource.getrlimit(resource.RLIMIT_STACK)
+        resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
+
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, env=env,
+                       preexec_fn=_stack_unlimited)
     if r.returncode != 0:
         raise RuntimeError(f"reference run failed ({r.returncode}): {r.stdout[-2000:]} {r.stderr[-2000:]}")
     for line in r.stdout.splitlines():

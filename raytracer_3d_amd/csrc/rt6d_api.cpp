@@ -85,7 +85,7 @@ struct rt6d_ctx {
     size_t vel_count[kAxes] = {0};
     bool params_ready = false;
     bool homog = false;
-    int opt_exact = 0, opt_path = 0, opt_block = 256;
+    int opt_exact = 0, opt_path = 0, opt_block = 256, opt_kernel = 0;
     void *d_hx = nullptr, *d_hy = nullptr, *d_hz = nullptr, *d_hm = nullptr, *d_hl = nullptr;
     uint32_t n4 = 0, inner4m = 0;
     double vmax[kAxes] = {0};
@@ -677,7 +677,7 @@ static int do_steps(rt6d_ctx *c, double dt, int nsteps)
     if (nsteps < 0 || !std::isfinite(dt)) return fail(RT6D_EINVAL, "rt6d_step: bad arguments");
     if (nsteps == 0) return 0;
     if ((r = ensure_coef(c, dt))) return r;
-    const LaunchCfg lc{c->opt_block, c->stream};
+    const LaunchCfg lc{c->opt_block, c->opt_kernel, c->stream};
     for (int s = 0; s < nsteps; ++s) {
         void *in = c->d_E[c->cur], *out = c->d_E[c->cur ^ 1];
         hipError_t e;
@@ -886,6 +886,36 @@ int rt6d_diagnose(rt6d_ctx *c, double dt, rt6d_diag *out, double *E_int, double 
     return 0;
 }
 
+int rt6d_copy_bandwidth(long long bytes, int reps, double *ms)
+{
+    if (bytes < 16 || reps < 1 || !ms) return fail(RT6D_EINVAL, "rt6d_copy_bandwidth: bad arguments");
+    const size_t nb = (size_t)bytes & ~(size_t)15;
+    void *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&a, nb);
+    if (e == hipSuccess) e = hipMalloc(&b, nb);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMemsetAsync(a, 0, nb, s);
+    if (e == hipSuccess) e = launch_copy(a, b, nb, s);   // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, s);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_copy(a, b, nb, s);
+    if (e == hipSuccess) e = hipEventRecord(e1, s);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float f = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&f, e0, e1);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (s) hipStreamDestroy(s);
+    if (a) hipFree(a);
+    if (b) hipFree(b);
+    if (e != hipSuccess) return fail(RT6D_EHIP, "rt6d_copy_bandwidth: %s", hipGetErrorString(e));
+    *ms = (double)f / reps;
+    return 0;
+}
+
 int rt6d_set_option(rt6d_ctx *c, int option, int value)
 {
     if (!c) return fail(RT6D_EINVAL, "null context");
@@ -903,6 +933,16 @@ int rt6d_set_option(rt6d_ctx *c, int option, int value)
     case RT6D_OPT_BLOCK:
         if (value < 64 || value > 1024 || (value & 63)) return fail(RT6D_EINVAL, "block %d", value);
         c->opt_block = value;
+        return 0;
+    case RT6D_OPT_KERNEL:
+        if (value < 0 || value > 6) return fail(RT6D_EINVAL, "kernel option %d", value);
+        if ((value == 2 || value == 3) && !march_fits(c->g))
+            return fail(RT6D_EINVAL, "marching kernel needs nx*ny <= 1024 (have %d)", c->n[0] * c->n[1]);
+        if ((value == 3 || value == 5) && c->n[2] % 2 != 0)
+            return fail(RT6D_EINVAL, "two z cells per thread need an even nz (have %d)", c->n[2]);
+        if (value == 6 && c->n[2] % 4 != 0)
+            return fail(RT6D_EINVAL, "four z cells per thread need nz divisible by 4 (have %d)", c->n[2]);
+        c->opt_kernel = value;
         return 0;
     default:
         return fail(RT6D_EINVAL, "unknown option %d", option);

@@ -423,6 +423,21 @@ __global__ void k_f2d(const float *__restrict__ in, double *__restrict__ out, si
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         out[i] = (double)in[i];
 }
+// plain streaming copy, 16 bytes per lane per iteration
+__global__ void __launch_bounds__(256) k_copy(const uint4 *__restrict__ in, uint4 *__restrict__ out, size_t n16)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = in[i];
+}
+hipError_t launch_copy(const void *in, void *out, size_t bytes, hipStream_t s)
+{
+    const size_t n16 = bytes / 16;
+    uint32_t nb = nblocks(n16, 256);
+    nb = nb > 8192 ? 8192 : (nb ? nb : 1);
+    hipLaunchKernelGGL(k_copy, dim3(nb), dim3(256), 0, s, (const uint4 *)in, (uint4 *)out, n16);
+    return hipGetLastError();
+}
+
 hipError_t launch_convert_d2f(const double *in, float *out, size_t n, hipStream_t s)
 {
     uint32_t nb = nblocks(n, 256);
@@ -442,13 +457,7 @@ hipError_t launch_convert_f2d(const float *in, double *out, size_t n, hipStream_
 
 hipError_t launch_step_exact(const Geo &g, const StepPtrs<double> &p, bool homog, const LaunchCfg &lc)
 {
-    const uint32_t nbl = nblocks(g.ncells, lc.block);
-    const uint32_t nbh = ((nbl + 7u) / 8u) * 8u;
-    if (homog)
-        hipLaunchKernelGGL((k_step<double, true, true>), dim3(nbh), dim3(lc.block), 0, lc.stream, g, p, nbl);
-    else
-        hipLaunchKernelGGL((k_step<double, true, false>), dim3(nbh), dim3(lc.block), 0, lc.stream, g, p, nbl);
-    return hipGetLastError();
+    return launch_step_impl<double, true>(g, p, homog, lc);
 }
 
 hipError_t launch_tendency_exact(const Geo &g, const StepPtrs<double> &p, bool homog, hipStream_t s)

@@ -12,13 +12,7 @@ static inline uint32_t nblocks(uint64_t n, int b) { return (uint32_t)((n + b - 1
 template <typename T>
 hipError_t launch_step_fast(const Geo &g, const StepPtrs<T> &p, bool homog, const LaunchCfg &lc)
 {
-    const uint32_t nbl = nblocks(g.ncells, lc.block);
-    const uint32_t nbh = ((nbl + 7u) / 8u) * 8u;
-    if (homog)
-        hipLaunchKernelGGL((k_step<T, false, true>), dim3(nbh), dim3(lc.block), 0, lc.stream, g, p, nbl);
-    else
-        hipLaunchKernelGGL((k_step<T, false, false>), dim3(nbh), dim3(lc.block), 0, lc.stream, g, p, nbl);
-    return hipGetLastError();
+    return launch_step_impl<T, false>(g, p, homog, lc);
 }
 
 template <typename T>

@@ -150,6 +150,9 @@ __global__ void __launch_bounds__(1024) k_step(const Geo g, const StepPtrs<T> p,
     }
 }
 
+#include "rt6d_march.h"
+#include "rt6d_column.h"
+
 // Tendency fields of one step (the by-products integrate leaves behind:
 // dE, meanfl, flux_x..flux_m), computed from the state the step started from.
 template <typename T, bool EXACT, bool HOMOG>
@@ -379,9 +382,93 @@ __global__ void __launch_bounds__(256) k_diag_columns(const Geo g, const StepPtr
 // ---------------------------------------------------------------- launchers
 
 struct LaunchCfg {
-    int block;
+    int block;          // threads per block of the per-cell kernel
+    int kernel;         // 0 auto, 1 per-cell kernel, 2 marching kernel
     hipStream_t stream;
 };
+
+inline bool march_fits(const Geo &g) { return g.n[0] * g.n[1] <= 1024; }
+
+// z cells per thread of the marching kernel for a kernel option
+inline int march_zc(const Geo &g, int kernel)
+{
+    if (kernel == 2) return 1;
+    return (g.n[2] % 2 == 0) ? 2 : 1;
+}
+
+
+template <typename T, bool EXACT, int ZC>
+hipError_t launch_march(const Geo &g, const StepPtrs<T> &p, bool homog, hipStream_t stream)
+{
+    const uint32_t nxy = (uint32_t)g.n[0] * g.n[1];
+    const uint32_t nthr = ((nxy + 63u) / 64u) * 64u;
+    MarchCfg mc;
+    mc.nzb = g.n[2] / ZC;
+    mc.dzb = make_fastdiv((uint32_t)mc.nzb);
+    const uint32_t nbl = (uint32_t)mc.nzb * g.n[3] * g.n[4];
+    const uint32_t nbh = ((nbl + 7u) / 8u) * 8u;
+    const size_t smem = 3 * ZC * (size_t)nthr * sizeof(T);
+    if (homog)
+        hipLaunchKernelGGL((k_step_march<T, EXACT, true, ZC>), dim3(nbh), dim3(nthr), smem, stream, g, p, mc, nbl);
+    else
+        hipLaunchKernelGGL((k_step_march<T, EXACT, false, ZC>), dim3(nbh), dim3(nthr), smem, stream, g, p, mc, nbl);
+    return hipGetLastError();
+}
+
+template <typename T, bool EXACT, int ZC>
+hipError_t launch_col(const Geo &g, const StepPtrs<T> &p, bool homog, hipStream_t stream)
+{
+    const uint32_t nxy = (uint32_t)g.n[0] * g.n[1];
+    ColCfg cc;
+    cc.nxyp = (int)(((nxy + 63u) / 64u) * 64u);
+    cc.nzb = g.n[2] / ZC;
+    cc.dxyp = make_fastdiv((uint32_t)cc.nxyp);
+    cc.dzb = make_fastdiv((uint32_t)cc.nzb);
+    const uint64_t nthreads = (uint64_t)cc.nxyp * cc.nzb * g.n[3] * g.n[4];
+    const uint32_t nbl = (uint32_t)((nthreads + 255) / 256);
+    const uint32_t nbh = ((nbl + 7u) / 8u) * 8u;
+    if (homog)
+        hipLaunchKernelGGL((k_step_col<T, EXACT, true, ZC>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
+    else
+        hipLaunchKernelGGL((k_step_col<T, EXACT, false, ZC>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
+    return hipGetLastError();
+}
+
+// z cells per thread of the column kernel: the largest of 4, 2, 1 dividing nz
+inline int col_zc(const Geo &g, int kernel)
+{
+    if (kernel == 4) return 1;
+    if (kernel == 5) return g.n[2] % 2 == 0 ? 2 : 1;
+    if (g.n[2] % 4 == 0) return 4;
+    return g.n[2] % 2 == 0 ? 2 : 1;
+}
+
+// One step.  kernel option: 0 auto (column kernel), 1 per-cell kernel,
+// 2/3 LDS marching kernel with 1/2 z cells per thread (nx*ny <= 1024),
+// 4/5/6 column kernel with 1/2/4 z cells per thread.
+template <typename T, bool EXACT>
+hipError_t launch_step_impl(const Geo &g, const StepPtrs<T> &p, bool homog, const LaunchCfg &lc)
+{
+    if ((lc.kernel == 2 || lc.kernel == 3) && march_fits(g)) {
+        if (march_zc(g, lc.kernel) == 2) return launch_march<T, EXACT, 2>(g, p, homog, lc.stream);
+        return launch_march<T, EXACT, 1>(g, p, homog, lc.stream);
+    }
+    // the column kernel addresses each array with 32-bit byte offsets
+    const bool col_fits = (uint64_t)g.ncells * 2u * sizeof(T) < (1ull << 32);
+    if (lc.kernel != 1 && col_fits) {
+        const int zc = col_zc(g, lc.kernel);
+        if (zc == 4) return launch_col<T, EXACT, 4>(g, p, homog, lc.stream);
+        if (zc == 2) return launch_col<T, EXACT, 2>(g, p, homog, lc.stream);
+        return launch_col<T, EXACT, 1>(g, p, homog, lc.stream);
+    }
+    const uint32_t nbl = (uint32_t)((g.ncells + lc.block - 1) / lc.block);
+    const uint32_t nbh = ((nbl + 7u) / 8u) * 8u;
+    if (homog)
+        hipLaunchKernelGGL((k_step<T, EXACT, true>), dim3(nbh), dim3(lc.block), 0, lc.stream, g, p, nbl);
+    else
+        hipLaunchKernelGGL((k_step<T, EXACT, false>), dim3(nbh), dim3(lc.block), 0, lc.stream, g, p, nbl);
+    return hipGetLastError();
+}
 
 // Implemented in rt6d_exact.hip (EXACT, double only) and rt6d_fast.hip.
 hipError_t launch_step_exact(const Geo &g, const StepPtrs<double> &p, bool homog, const LaunchCfg &lc);
@@ -429,6 +516,7 @@ hipError_t launch_vmax(const Geo &g, const double *const vel[kAxes], unsigned lo
 hipError_t launch_cfl_max(const Geo &g, const double *const vel[kAxes], const AxisCoef &ax,
                           unsigned long long *bits, hipStream_t s);
 hipError_t launch_convert_d2f(const double *in, float *out, size_t n, hipStream_t s);
+hipError_t launch_copy(const void *in, void *out, size_t bytes, hipStream_t s);
 hipError_t launch_convert_f2d(const float *in, double *out, size_t n, hipStream_t s);
 
 }  // namespace rt6d

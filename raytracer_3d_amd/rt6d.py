@@ -30,14 +30,14 @@ FIELD = {"E": 0, "dE": 1, "meanfl": 2, "omega_i": 3, "omega_e": 4,
          "flux_x": 11, "flux_y": 12, "flux_z": 13, "flux_k": 14, "flux_l": 15, "flux_m": 16,
          "lambda_x": 17, "lambda_y": 18, "lambda_z": 19, "lambda": 20}
 LAYOUT_INTERIOR, LAYOUT_FACES, LAYOUT_HALO = 0, 1, 2
-OPT_EXACT, OPT_PATH, OPT_BLOCK = 1, 2, 3
+OPT_EXACT, OPT_PATH, OPT_BLOCK, OPT_KERNEL = 1, 2, 3, 4
 Q_HOMOGENEOUS, Q_DEVICE_BYTES, Q_CELLS, Q_STEPS, Q_BYTES_PER_CELL = 1, 2, 3, 4, 5
 
 # every symbol include/rt6d.h declares
 EXPORTS = ("rt6d_version", "rt6d_last_error", "rt6d_device_count", "rt6d_create", "rt6d_destroy",
            "rt6d_build_grid", "rt6d_set_grid", "rt6d_set_background", "rt6d_calc_parameter", "rt6d_cfl_dt",
            "rt6d_upload_E", "rt6d_step", "rt6d_step_timed", "rt6d_fluxes", "rt6d_download", "rt6d_diagnose",
-           "rt6d_set_option", "rt6d_query", "rt6d_synchronize")
+           "rt6d_set_option", "rt6d_query", "rt6d_synchronize", "rt6d_copy_bandwidth")
 
 
 class Diag(C.Structure):
@@ -93,6 +93,7 @@ def load_library(path: str | None = None):
         "rt6d_set_option": ([vp, C.c_int, C.c_int], C.c_int),
         "rt6d_query": ([vp, C.c_int, C.POINTER(C.c_longlong)], C.c_int),
         "rt6d_synchronize": ([vp], C.c_int),
+        "rt6d_copy_bandwidth": ([C.c_longlong, C.c_int, dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -119,6 +120,14 @@ def _check(lib, rc: int, what: str):
 
 def device_count() -> int:
     return load_library().rt6d_device_count()
+
+
+def device_copy_timed(nbytes: int, reps: int = 10) -> float:
+    """Average ms of a plain device copy of nbytes (rt6d_copy_bandwidth)."""
+    lib = load_library()
+    ms = C.c_double()
+    _check(lib, lib.rt6d_copy_bandwidth(int(nbytes), reps, C.byref(ms)), "rt6d_copy_bandwidth")
+    return ms.value
 
 
 def build_grid(axis: int, spacing, fixed: float = 0.0):
@@ -184,7 +193,8 @@ class Model:
         _check(self.lib, self.lib.rt6d_upload_E(self._h, _dptr(E), layout), "rt6d_upload_E")
 
     @classmethod
-    def from_case(cls, case: "_cases.Case", precision=FP64, exact=False, path=0, block=None) -> "Model":
+    def from_case(cls, case: "_cases.Case", precision=FP64, exact=False, path=0, block=None,
+                  kernel=0) -> "Model":
         m = cls(case.n, case.reflect_up, case.reflect_down, precision)
         if exact:
             m.set_option(OPT_EXACT, 1)
@@ -192,6 +202,8 @@ class Model:
             m.set_option(OPT_PATH, path)
         if block:
             m.set_option(OPT_BLOCK, block)
+        if kernel:
+            m.set_option(OPT_KERNEL, kernel)
         for ax, (t, dt) in enumerate(case.grids()):
             m.set_grid(ax, t, dt)
         m.set_background(case.Nsqr, case.U, case.V, case.f0, case.sign)

@@ -1,0 +1,67 @@
+"""Time one step configuration of the library (for profiling under rocprofv3).
+
+    python -m raytracer_3d_amd.stepbench --workload gm --precision fp64 \
+        --path auto --kernel auto --steps 20 --warmup 3 [--copy-mib 1024]
+
+Workloads: ``gm`` (24^3 x 16^3 Garrett-Munk-shaped spectrum, the production
+size), ``gm32`` (32^3 x 16^3), ``packet`` (32x4x32x16x4x16 wave packet).
+``--copy-mib`` first runs a plain device-to-device copy of that size through
+librt6d's own copy kernel, the reference point for the HBM byte counters.
+Prints one JSON line with the event-timed launch average.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+from . import cases, rt6d
+
+WORKLOADS = {
+    "gm": lambda: cases.gm_case(),
+    "gm32": lambda: cases.gm_case(n=(32, 32, 32, 16, 16, 16)),
+    "packet": lambda: cases.packet_case(),
+}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="gm", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--path", default="auto", choices=["auto", "general"])
+    ap.add_argument("--kernel", default="auto", choices=["auto", "percell", "march1", "march2", "col1", "col2", "col4"])
+    ap.add_argument("--exact", action="store_true")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--copy-mib", type=int, default=0)
+    a = ap.parse_args(argv)
+    if a.copy_mib:
+        ms = rt6d.device_copy_timed(a.copy_mib << 20)
+        print(json.dumps({"copy_bytes": a.copy_mib << 20, "copy_ms": ms}), flush=True)
+    case = WORKLOADS[a.workload]()
+    prec = rt6d.FP64 if a.precision == "fp64" else rt6d.FP32
+    kern = {"auto": 0, "percell": 1, "march1": 2, "march2": 3, "col1": 4, "col2": 5, "col4": 6}[a.kernel]
+    m = rt6d.Model.from_case(case, precision=prec, path=1 if a.path == "general" else 0, kernel=kern,
+                             exact=a.exact)
+    m.calc_parameter()
+    dt = m.cfl_dt(0.5)
+    m.integrate(a.warmup, dt=dt)
+    m.synchronize()
+    t0 = time.perf_counter()
+    ms = m.integrate_timed(a.steps, dt=dt)
+    t1 = time.perf_counter()
+    out = {
+        "workload": a.workload, "n": list(case.n), "precision": a.precision, "path": a.path, "kernel": a.kernel,
+        "exact": a.exact, "homogeneous": m.homogeneous, "steps": a.steps, "launch_ms": ms / a.steps,
+        "wall_ms_per_step": 1e3 * (t1 - t0) / a.steps, "bytes_per_cell": m.bytes_per_cell,
+        "cell_updates_per_s": case.cells * a.steps / (t1 - t0),
+    }
+    out["achieved_GBs"] = out["bytes_per_cell"] * case.cells / (out["launch_ms"] * 1e-3) / 1e9
+    print(json.dumps(out), flush=True)
+    m.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

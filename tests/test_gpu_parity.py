@@ -44,13 +44,16 @@ def test_calc_parameter_bitwise(entry, gpu_available):
         assert m.homogeneous == (entry["stem"] in HOMOG_EXPECTED)
 
 
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3, 4, 5], ids=["auto", "percell", "march1", "march2", "col1", "col2"])
 @pytest.mark.parametrize("exact", [True, False], ids=["exact", "fast"])
 @pytest.mark.parametrize("path", [0, 1], ids=["auto", "general"])
 @pytest.mark.parametrize("entry", ENTRIES, ids=IDS)
-def test_integrate_matches_reference(entry, path, exact, gpu_available):
+def test_integrate_matches_reference(entry, path, exact, kernel, gpu_available):
     case = golden_case(entry)
     g = golden_data(entry)
-    with _model(case, exact=exact, path=path) as m:
+    if kernel in (3, 5) and case.n[2] % 2:
+        pytest.skip("two z cells per thread need an even nz")
+    with _model(case, exact=exact, path=path, kernel=kernel) as m:
         if path == 1:
             assert not m.homogeneous
         # bitwise equality is expected in EXACT mode where lambda_x = lambda_y = 0
@@ -182,7 +185,18 @@ def test_full_size_determinism_paths_and_budget(gm_full, gpu_available):
         assert rel_l2(m.download("E"), e1) <= 1e-14
     with _model(c, exact=True) as m:
         m.integrate(10, dt=dt)
-        assert rel_l2(m.download("E"), e1) <= TOL64
+        ex = m.download("E")
+        assert rel_l2(ex, e1) <= TOL64
+    with _model(c, exact=True, kernel=1) as m:
+        m.integrate(10, dt=dt)
+        assert np.array_equal(m.download("E"), ex), "marching and per-cell kernels differ in exact mode"
+    for kernel in (1, 2, 3, 4, 5, 6):
+        with _model(c, kernel=kernel) as m:
+            m.integrate(10, dt=dt)
+            assert rel_l2(m.download("E"), e1) <= 1e-14, kernel
+        with _model(c, exact=True, kernel=kernel) as m:
+            m.integrate(10, dt=dt)
+            assert np.array_equal(m.download("E"), ex), f"kernel {kernel} differs in exact mode"
     terms = [d["eint"], d["mflint"]] + [d[k] for k in d if k.startswith("bfl_")]
     assert abs(d["residual"]) <= 1e-11 * sum(abs(t) for t in terms)
     assert np.all(np.isfinite(e1))
