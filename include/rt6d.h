@@ -63,8 +63,8 @@ extern "C" {
 #define RT6D_OPT_PATH 2      /* 0 auto (default), 1 force the general path
                                 (6D coefficient fields streamed from HBM) */
 #define RT6D_OPT_BLOCK 3     /* threads per block of the per-cell step kernel (64..1024) */
-#define RT6D_OPT_KERNEL 4    /* step kernel: 0 auto (default, the column kernel with the most
-                                z cells per thread that divide nz); 1 per-cell kernel;
+#define RT6D_OPT_KERNEL 4    /* step kernel: 0 auto (default, the column kernel with two z
+                                cells per thread when nz is even); 1 per-cell kernel;
                                 2/3 LDS marching kernel, 1/2 z cells per thread (nx*ny <= 1024);
                                 4/5/6 column kernel, 1/2/4 z cells per thread */
 
@@ -172,6 +172,35 @@ int rt6d_copy_bandwidth(long long bytes, int reps, double *ms);
 int rt6d_set_option(rt6d_ctx *ctx, int option, int value);
 int rt6d_query(rt6d_ctx *ctx, int what, long long *value);
 int rt6d_synchronize(rt6d_ctx *ctx);
+
+/*
+ * Ensembles (no reference counterpart: the reference runs one integration
+ * per MPI job).  An ensemble advances many independent integrations -- each
+ * a context set up as above (calc_parameter done, E uploaded), with its own
+ * grid, background and dt -- in one persistent kernel launch per step over a
+ * (member, block) work list, the steps replayed from a captured HIP graph.
+ * Members keep their own state; while in an ensemble a context must not be
+ * stepped on its own.  rt6d_ensemble_remove (or destroy) hands the context
+ * back with its state current, so rt6d_download/rt6d_diagnose work on it.
+ * All members share the ensemble's precision and device.
+ */
+typedef struct rt6d_ensemble rt6d_ensemble;
+int rt6d_ensemble_create(int precision, rt6d_ensemble **out);
+int rt6d_ensemble_destroy(rt6d_ensemble *ens);
+/* add a member stepping with dt; *slot receives its slot number */
+int rt6d_ensemble_add(rt6d_ensemble *ens, rt6d_ctx *member, double dt, int *slot);
+/* take a member out; its context reflects every step the ensemble took */
+int rt6d_ensemble_remove(rt6d_ensemble *ens, int slot);
+/* advance every member nsteps steps; *ms (may be NULL) gets the device time */
+int rt6d_ensemble_step(rt6d_ensemble *ens, int nsteps, double *ms);
+/* schedule: persistent blocks per compute unit (1..16) and work-list order
+ * (0: member by member, 1: members interleaved block by block) */
+int rt6d_ensemble_set_schedule(rt6d_ensemble *ens, int blocks_per_cu, int order);
+/* autotuner: times each schedule on `trial_steps` real steps of the current
+ * members (results do not depend on the schedule) and keeps the fastest */
+int rt6d_ensemble_tune(rt6d_ensemble *ens, int trial_steps, int *blocks_per_cu, int *order);
+/* number of members, total cells, steps taken by the ensemble */
+int rt6d_ensemble_info(rt6d_ensemble *ens, int *members, long long *cells, long long *steps);
 
 #ifdef __cplusplus
 }

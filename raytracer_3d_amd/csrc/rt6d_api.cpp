@@ -99,6 +99,7 @@ struct rt6d_ctx {
     size_t dev_bytes = 0;
     int *d_flag = nullptr;
     unsigned long long *d_bits = nullptr;
+    rt6d_ensemble *owner = nullptr;  // ensemble this context is a member of
 
     int dalloc(void **p, size_t bytes)
     {
@@ -202,6 +203,15 @@ StepPtrs<T> step_ptrs(const rt6d_ctx *c, const void *Ein, void *Eout, double dt)
     for (int d = 0; d < kAxes; ++d) p.flux[d] = (T *)c->d_tfl[d];
     return p;
 }
+
+// a context that belongs to an ensemble is stepped only by the ensemble
+int not_member(const rt6d_ctx *c, const char *fn)
+{
+    if (c->owner) return fail(RT6D_ESTATE, "%s: context is an ensemble member (remove it first)", fn);
+    return 0;
+}
+
+void ens_detach(rt6d_ensemble *e, rt6d_ctx *c);
 
 int require_ready(rt6d_ctx *c)
 {
@@ -454,6 +464,7 @@ int rt6d_create(const int *n, int reflect_up, int reflect_down, int precision, r
 int rt6d_destroy(rt6d_ctx *c)
 {
     if (!c) return 0;
+    if (c->owner) ens_detach(c->owner, c);
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     c->dfree((void *&)c->d_grid);
@@ -489,6 +500,7 @@ int rt6d_set_grid(rt6d_ctx *c, int axis, const double *t, const double *dt)
 {
     int r;
     if ((r = check_ctx(c))) return r;
+    if ((r = not_member(c, "rt6d_set_grid"))) return r;
     if (axis < 0 || axis >= kAxes || !t || !dt) return fail(RT6D_EINVAL, "rt6d_set_grid: bad arguments");
     const int N = c->n[axis] + 4;
     for (int p = 0; p < N; ++p) {
@@ -512,6 +524,7 @@ int rt6d_set_background(rt6d_ctx *c, const double *Nsqr, const double *U, const 
 {
     int r;
     if ((r = check_ctx(c))) return r;
+    if ((r = not_member(c, "rt6d_set_background"))) return r;
     if (!Nsqr || !U || !V) return fail(RT6D_EINVAL, "rt6d_set_background: null array");
     c->f0 = f0;
     c->sign = sign_of_omega_i;
@@ -529,6 +542,7 @@ int rt6d_calc_parameter(rt6d_ctx *c)
 {
     int r;
     if ((r = check_ctx(c))) return r;
+    if ((r = not_member(c, "rt6d_calc_parameter"))) return r;
     for (int d = 0; d < kAxes; ++d)
         if (!c->grid_set[d]) return fail(RT6D_ESTATE, "rt6d_calc_parameter: grid of axis %d not set", d);
     if (!c->bg_set) return fail(RT6D_ESTATE, "rt6d_calc_parameter: background not set");
@@ -644,6 +658,7 @@ int rt6d_upload_E(rt6d_ctx *c, const double *E, int layout)
 {
     int r;
     if ((r = check_ctx(c))) return r;
+    if ((r = not_member(c, "rt6d_upload_E"))) return r;
     if (!E) return fail(RT6D_EINVAL, "rt6d_upload_E: null array");
     const double *src = E;
     std::vector<double> tmp;
@@ -673,6 +688,7 @@ int rt6d_upload_E(rt6d_ctx *c, const double *E, int layout)
 static int do_steps(rt6d_ctx *c, double dt, int nsteps)
 {
     int r;
+    if ((r = not_member(c, "rt6d_step"))) return r;
     if ((r = require_ready(c))) return r;
     if (nsteps < 0 || !std::isfinite(dt)) return fail(RT6D_EINVAL, "rt6d_step: bad arguments");
     if (nsteps == 0) return 0;
@@ -919,6 +935,7 @@ int rt6d_copy_bandwidth(long long bytes, int reps, double *ms)
 int rt6d_set_option(rt6d_ctx *c, int option, int value)
 {
     if (!c) return fail(RT6D_EINVAL, "null context");
+    if (int r = not_member(c, "rt6d_set_option")) return r;
     switch (option) {
     case RT6D_OPT_EXACT:
         if (value && c->prec != RT6D_FP64) return fail(RT6D_EINVAL, "exact arithmetic needs RT6D_FP64");
@@ -977,6 +994,340 @@ int rt6d_synchronize(rt6d_ctx *c)
     int r;
     if ((r = check_ctx(c))) return r;
     HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+
+// ================================================================= ensembles
+
+}  // extern "C"
+
+struct rt6d_ensemble {
+    int prec = RT6D_FP64;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<rt6d_ctx *> slot;     // nullptr = free
+    std::vector<double> dt;
+    std::vector<int> cur0;            // member's current buffer at the last rebuild
+    int blocks_per_cu = 4, order = 0, ncu = 256;
+    void *d_mem = nullptr;
+    size_t mem_cap = 0;
+    EnsTask *d_tasks = nullptr;
+    size_t task_cap = 0;
+    uint32_t ntasks = 0;
+    bool dirty = true;
+    long long k = 0;                  // steps since the last rebuild
+    long long steps = 0;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};   // two steps starting at parity 0 / 1
+};
+
+namespace {
+
+template <typename T>
+int ens_rebuild_t(rt6d_ensemble *e)
+{
+    std::vector<EnsMember<T>> mem;
+    std::vector<uint32_t> nbl;
+    for (size_t i = 0; i < e->slot.size(); ++i) {
+        EnsMember<T> M{};
+        rt6d_ctx *c = e->slot[i];
+        if (c) {
+            int zc = 1;
+            if (!col_config<T>(c->g, M.cc, zc)) return fail(RT6D_EINVAL, "ensemble member too large");
+            M.g = c->g;
+            const int a = c->cur;
+            M.p[0] = step_ptrs<T>(c, c->d_E[a], c->d_E[a ^ 1], e->dt[i]);
+            M.p[1] = step_ptrs<T>(c, c->d_E[a ^ 1], c->d_E[a], e->dt[i]);
+            M.variant = (c->homog ? 1u : 0u) | (zc == 2 ? 2u : 0u);
+            e->cur0[i] = a;
+            nbl.push_back(col_blocks<T>(c->g, M.cc));
+        } else {
+            nbl.push_back(0);
+        }
+        mem.push_back(M);
+    }
+    std::vector<EnsTask> tasks;
+    if (e->order == 0) {
+        for (size_t i = 0; i < nbl.size(); ++i)
+            for (uint32_t b = 0; b < nbl[i]; ++b) tasks.push_back(EnsTask{(uint32_t)i, b});
+    } else {
+        uint32_t mx = 0;
+        for (uint32_t v : nbl) mx = v > mx ? v : mx;
+        for (uint32_t b = 0; b < mx; ++b)
+            for (size_t i = 0; i < nbl.size(); ++i)
+                if (b < nbl[i]) tasks.push_back(EnsTask{(uint32_t)i, b});
+    }
+    const size_t mb = mem.size() * sizeof(EnsMember<T>), tb = tasks.size() * sizeof(EnsTask);
+    if (mb > e->mem_cap) {
+        if (e->d_mem) hipFree(e->d_mem);
+        e->d_mem = nullptr;
+        HIPCHK(hipMalloc(&e->d_mem, mb));
+        e->mem_cap = mb;
+    }
+    if (tb > e->task_cap) {
+        if (e->d_tasks) hipFree(e->d_tasks);
+        e->d_tasks = nullptr;
+        HIPCHK(hipMalloc((void **)&e->d_tasks, tb));
+        e->task_cap = tb;
+    }
+    if (mb) HIPCHK(hipMemcpyAsync(e->d_mem, mem.data(), mb, hipMemcpyHostToDevice, e->stream));
+    if (tb) HIPCHK(hipMemcpyAsync(e->d_tasks, tasks.data(), tb, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->ntasks = (uint32_t)tasks.size();
+    for (int q = 0; q < 2; ++q)
+        if (e->graph[q]) { hipGraphExecDestroy(e->graph[q]); e->graph[q] = nullptr; }
+    e->k = 0;
+    e->dirty = false;
+    return 0;
+}
+
+// write the ensemble's progress back into the member contexts
+void ens_sync_members(rt6d_ensemble *e)
+{
+    for (size_t i = 0; i < e->slot.size(); ++i) {
+        rt6d_ctx *c = e->slot[i];
+        if (!c) continue;
+        c->cur = e->cur0[i] ^ (int)(e->k & 1);
+    }
+}
+
+int ens_launch(rt6d_ensemble *e, uint32_t parity)
+{
+    const uint32_t nchunks = (uint32_t)(e->blocks_per_cu * e->ncu);
+    hipError_t r;
+    if (e->prec == RT6D_FP64)
+        r = launch_ensemble<double>((const EnsMember<double> *)e->d_mem, e->d_tasks, e->ntasks, parity, nchunks,
+                                    e->stream);
+    else
+        r = launch_ensemble<float>((const EnsMember<float> *)e->d_mem, e->d_tasks, e->ntasks, parity, nchunks,
+                                   e->stream);
+    HIPCHK(r);
+    return 0;
+}
+
+int ens_graph(rt6d_ensemble *e, int start)
+{
+    if (e->graph[start]) return 0;
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    int r1 = ens_launch(e, (uint32_t)start);
+    int r2 = r1 ? r1 : ens_launch(e, (uint32_t)(start ^ 1));
+    hipError_t ec = hipStreamEndCapture(e->stream, &g);
+    if (r2) {
+        if (g) hipGraphDestroy(g);
+        return r2;
+    }
+    HIPCHK(ec);
+    hipError_t ei = hipGraphInstantiate(&e->graph[start], g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    HIPCHK(ei);
+    return 0;
+}
+
+int ens_steps(rt6d_ensemble *e, int nsteps)
+{
+    int r;
+    if (e->dirty) {
+        ens_sync_members(e);
+        r = e->prec == RT6D_FP64 ? ens_rebuild_t<double>(e) : ens_rebuild_t<float>(e);
+        if (r) return r;
+    }
+    if (e->ntasks == 0) return 0;
+    // a member's dx/dt coefficients follow the last dt used on it (diagnose may use another)
+    for (size_t i = 0; i < e->slot.size(); ++i)
+        if (e->slot[i] && (r = ensure_coef(e->slot[i], e->dt[i]))) return r;
+    int left = nsteps;
+    while (left >= 2) {
+        const int start = (int)(e->k & 1);
+        if ((r = ens_graph(e, start))) return r;
+        HIPCHK(hipGraphLaunch(e->graph[start], e->stream));
+        e->k += 2;
+        left -= 2;
+    }
+    if (left == 1) {
+        if ((r = ens_launch(e, (uint32_t)(e->k & 1)))) return r;
+        e->k += 1;
+    }
+    e->steps += nsteps;
+    for (size_t i = 0; i < e->slot.size(); ++i) {
+        rt6d_ctx *c = e->slot[i];
+        if (!c) continue;
+        c->steps += nsteps;
+        c->have_prev = true;
+        c->prev_dt = e->dt[i];
+        c->fl_src = c->de_src = 0;
+    }
+    ens_sync_members(e);
+    return 0;
+}
+
+// take c out of e (its context is being destroyed)
+void ens_detach(rt6d_ensemble *e, rt6d_ctx *c)
+{
+    for (size_t i = 0; i < e->slot.size(); ++i)
+        if (e->slot[i] == c) rt6d_ensemble_remove(e, (int)i);
+}
+
+}  // namespace
+
+extern "C" {
+
+int rt6d_ensemble_create(int precision, rt6d_ensemble **out)
+{
+    if (!out) return fail(RT6D_EINVAL, "rt6d_ensemble_create: null output");
+    if (precision != RT6D_FP64 && precision != RT6D_FP32) return fail(RT6D_EINVAL, "precision %d", precision);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(RT6D_EHIP, "rt6d_ensemble_create: no HIP device");
+    rt6d_ensemble *e = new rt6d_ensemble();
+    e->prec = precision;
+    hipGetDevice(&e->device);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, e->device) == hipSuccess && prop.multiProcessorCount > 0)
+        e->ncu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess) {
+        rt6d_ensemble_destroy(e);
+        return fail(RT6D_EHIP, "rt6d_ensemble_create: stream/event creation failed");
+    }
+    *out = e;
+    return 0;
+}
+
+int rt6d_ensemble_destroy(rt6d_ensemble *e)
+{
+    if (!e) return 0;
+    hipSetDevice(e->device);
+    if (e->stream) hipStreamSynchronize(e->stream);
+    ens_sync_members(e);
+    for (rt6d_ctx *c : e->slot)
+        if (c) c->owner = nullptr;
+    for (int q = 0; q < 2; ++q)
+        if (e->graph[q]) hipGraphExecDestroy(e->graph[q]);
+    if (e->d_mem) hipFree(e->d_mem);
+    if (e->d_tasks) hipFree(e->d_tasks);
+    if (e->ev0) hipEventDestroy(e->ev0);
+    if (e->ev1) hipEventDestroy(e->ev1);
+    if (e->stream) hipStreamDestroy(e->stream);
+    delete e;
+    return 0;
+}
+
+int rt6d_ensemble_add(rt6d_ensemble *e, rt6d_ctx *c, double dt, int *slot)
+{
+    if (!e || !c) return fail(RT6D_EINVAL, "rt6d_ensemble_add: null argument");
+    if (c->prec != e->prec) return fail(RT6D_EINVAL, "rt6d_ensemble_add: member precision differs");
+    if (c->device != e->device) return fail(RT6D_EINVAL, "rt6d_ensemble_add: member on another device");
+    if (!(dt > 0.0) || !std::isfinite(dt)) return fail(RT6D_EINVAL, "rt6d_ensemble_add: dt = %g", dt);
+    int r;
+    if ((r = check_ctx(c))) return r;
+    if ((r = require_ready(c))) return r;
+    if (c->owner) return fail(RT6D_ESTATE, "rt6d_ensemble_add: context already in an ensemble");
+    if (c->opt_exact) return fail(RT6D_EINVAL, "rt6d_ensemble_add: ensembles run the fast kernels (exact option set)");
+    ColCfg cc;
+    int zc;
+    const bool fits = e->prec == RT6D_FP64 ? col_config<double>(c->g, cc, zc) : col_config<float>(c->g, cc, zc);
+    if (!fits) return fail(RT6D_EINVAL, "rt6d_ensemble_add: member too large for the ensemble kernel");
+    if ((r = ensure_coef(c, dt))) return r;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    // finish pending ensemble steps before the member set changes
+    HIPCHK(hipStreamSynchronize(e->stream));
+    ens_sync_members(e);
+    int s = -1;
+    for (size_t i = 0; i < e->slot.size(); ++i)
+        if (!e->slot[i]) { s = (int)i; break; }
+    if (s < 0) {
+        s = (int)e->slot.size();
+        e->slot.push_back(nullptr);
+        e->dt.push_back(0.0);
+        e->cur0.push_back(0);
+    }
+    e->slot[s] = c;
+    e->dt[s] = dt;
+    e->cur0[s] = c->cur ^ (int)(e->k & 1);  // so that ens_sync_members leaves c->cur as it is
+    c->owner = e;
+    e->dirty = true;
+    if (slot) *slot = s;
+    return 0;
+}
+
+int rt6d_ensemble_remove(rt6d_ensemble *e, int s)
+{
+    if (!e || s < 0 || s >= (int)e->slot.size() || !e->slot[s])
+        return fail(RT6D_EINVAL, "rt6d_ensemble_remove: bad slot %d", s);
+    HIPCHK(hipStreamSynchronize(e->stream));
+    ens_sync_members(e);
+    e->slot[s]->owner = nullptr;
+    e->slot[s] = nullptr;
+    e->dirty = true;
+    return 0;
+}
+
+int rt6d_ensemble_step(rt6d_ensemble *e, int nsteps, double *ms)
+{
+    if (!e || nsteps < 0) return fail(RT6D_EINVAL, "rt6d_ensemble_step: bad arguments");
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipEventRecord(e->ev0, e->stream));
+    int r = ens_steps(e, nsteps);
+    if (r) return r;
+    HIPCHK(hipEventRecord(e->ev1, e->stream));
+    HIPCHK(hipEventSynchronize(e->ev1));
+    if (ms) {
+        float f = 0.f;
+        HIPCHK(hipEventElapsedTime(&f, e->ev0, e->ev1));
+        *ms = (double)f;
+    }
+    return 0;
+}
+
+int rt6d_ensemble_set_schedule(rt6d_ensemble *e, int blocks_per_cu, int order)
+{
+    if (!e || blocks_per_cu < 1 || blocks_per_cu > 16 || order < 0 || order > 1)
+        return fail(RT6D_EINVAL, "rt6d_ensemble_set_schedule: bad arguments");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    ens_sync_members(e);
+    if (order != e->order) e->dirty = true;
+    if (blocks_per_cu != e->blocks_per_cu) {
+        for (int q = 0; q < 2; ++q)
+            if (e->graph[q]) { hipGraphExecDestroy(e->graph[q]); e->graph[q] = nullptr; }
+    }
+    e->blocks_per_cu = blocks_per_cu;
+    e->order = order;
+    return 0;
+}
+
+int rt6d_ensemble_tune(rt6d_ensemble *e, int trial_steps, int *bpc_out, int *order_out)
+{
+    if (!e || trial_steps < 2) return fail(RT6D_EINVAL, "rt6d_ensemble_tune: need >= 2 trial steps");
+    const int bpcs[] = {2, 4, 8};
+    double best = 1e300;
+    int bb = e->blocks_per_cu, bo = e->order;
+    for (int order = 0; order <= 1; ++order)
+        for (int bpc : bpcs) {
+            int r = rt6d_ensemble_set_schedule(e, bpc, order);
+            if (r) return r;
+            double ms = 0.0;
+            if ((r = rt6d_ensemble_step(e, 2, nullptr))) return r;   // build + warm the graph
+            if ((r = rt6d_ensemble_step(e, trial_steps, &ms))) return r;
+            if (ms < best) { best = ms; bb = bpc; bo = order; }
+        }
+    int r = rt6d_ensemble_set_schedule(e, bb, bo);
+    if (r) return r;
+    if (bpc_out) *bpc_out = bb;
+    if (order_out) *order_out = bo;
+    return 0;
+}
+
+int rt6d_ensemble_info(rt6d_ensemble *e, int *members, long long *cells, long long *steps)
+{
+    if (!e) return fail(RT6D_EINVAL, "null ensemble");
+    int n = 0;
+    long long cc = 0;
+    for (rt6d_ctx *c : e->slot)
+        if (c) { ++n; cc += (long long)c->cells; }
+    if (members) *members = n;
+    if (cells) *cells = cc;
+    if (steps) *steps = e->steps;
     return 0;
 }
 

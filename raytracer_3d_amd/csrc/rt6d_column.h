@@ -63,11 +63,10 @@ __device__ __forceinline__ void bst<float>(float v, __amdgpu_buffer_rsrc_t r, ui
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, (int)voff, (int)soff, 0);
 }
 
+// The work of logical block b (256 threads) of one integration.
 template <typename T, bool EXACT, bool HOMOG, int ZC>
-__global__ void __launch_bounds__(256) k_step_col(const Geo g, const StepPtrs<T> p, const ColCfg cc, uint32_t nbl)
+__device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, const ColCfg &cc, uint32_t b)
 {
-    const uint32_t b = xcd_logical_block(blockIdx.x, nbl);
-    if (b >= nbl) return;
     const uint32_t t = b * blockDim.x + threadIdx.x;
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2], nk = g.n[3], nl = g.n[4], nm = g.n[5];
     const int nxy = nx * ny;
@@ -313,4 +312,12 @@ __global__ void __launch_bounds__(256) k_step_col(const Geo g, const StepPtrs<T>
 #pragma unroll
         for (int q = 0; q < 4; ++q) zq[q] = znx[q];
     }
+}
+
+template <typename T, bool EXACT, bool HOMOG, int ZC>
+__global__ void __launch_bounds__(256) k_step_col(const Geo g, const StepPtrs<T> p, const ColCfg cc, uint32_t nbl)
+{
+    const uint32_t b = xcd_logical_block(blockIdx.x, nbl);
+    if (b >= nbl) return;
+    col_body<T, EXACT, HOMOG, ZC>(g, p, cc, b);
 }

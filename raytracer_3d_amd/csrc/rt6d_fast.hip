@@ -47,6 +47,20 @@ hipError_t launch_diag_fast(const Geo &g, const StepPtrs<T> &p, bool homog, cons
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_ensemble(const EnsMember<T> *mem, const EnsTask *tasks, uint32_t ntasks, uint32_t parity,
+                           uint32_t nchunks, hipStream_t s)
+{
+    if (ntasks == 0) return hipSuccess;
+    const uint32_t nbh = ((nchunks + 7u) / 8u) * 8u;
+    hipLaunchKernelGGL(k_ens_step<T>, dim3(nbh), dim3(256), 0, s, mem, tasks, ntasks, parity, nchunks);
+    return hipGetLastError();
+}
+template hipError_t launch_ensemble<double>(const EnsMember<double> *, const EnsTask *, uint32_t, uint32_t, uint32_t,
+                                            hipStream_t);
+template hipError_t launch_ensemble<float>(const EnsMember<float> *, const EnsTask *, uint32_t, uint32_t, uint32_t,
+                                           hipStream_t);
+
 template hipError_t launch_step_fast<double>(const Geo &, const StepPtrs<double> &, bool, const LaunchCfg &);
 template hipError_t launch_step_fast<float>(const Geo &, const StepPtrs<float> &, bool, const LaunchCfg &);
 template hipError_t launch_tendency_fast<double>(const Geo &, const StepPtrs<double> &, bool, hipStream_t);

@@ -1,0 +1,223 @@
+! rt6d_hot.f90 -- the reference's hot-path routines under their own names
+! and argument-less signatures, forwarding to the GPU through rt6d_abi:
+!   calc_parameter   for_src/calc_parameter.f90:2
+!   integrate        for_src/integrate.f90:4
+!   superbee_fluxes  for_src/superbee_fluxes.f90:4
+!   diagnose         for_src/diagnose.f90:20 (reductions and printout; no netCDF)
+! plus the host set-up the driver keeps (pe_decomposition for one device,
+! allocate_main_module, calc_grid) and halt_stop.
+
+subroutine halt_stop(string)
+  implicit none
+  character*(*) :: string
+  print *, ' halting: ', trim(string)
+  stop 99
+end subroutine halt_stop
+
+subroutine rt6d_check(rc, where)
+  use iso_c_binding, only: c_int
+  use rt6d_abi, only: rt6d_error_message
+  implicit none
+  integer(c_int), intent(in) :: rc
+  character*(*), intent(in) :: where
+  if (rc /= 0) then
+    print *, ' rt6d error in ', trim(where), ': ', trim(rt6d_error_message())
+    call halt_stop(where)
+  endif
+end subroutine rt6d_check
+
+! One GPU owns the whole grid (parallel_mpi.f90:153-162, the n_pes = 1 branch).
+subroutine pe_decomposition
+  use main_module
+  implicit none
+  if (nk > 1 .and. mod(nk, 2) /= 0) call halt_stop(' nk must be even or one ')
+  if (nl > 1 .and. mod(nl, 2) /= 0) call halt_stop(' nl must be even or one ')
+  if (nm > 1 .and. mod(nm, 2) /= 0) call halt_stop(' nm must be even or one')
+  x_blk = nx; y_blk = ny; z_blk = nz; k_blk = nk; l_blk = nl; m_blk = nm
+  xs_pe = 1; xe_pe = nx; ys_pe = 1; ye_pe = ny; zs_pe = 1; ze_pe = nz
+  ks_pe = 1; ke_pe = nk; ls_pe = 1; le_pe = nl; ms_pe = 1; me_pe = nm
+end subroutine pe_decomposition
+
+! Host arrays (halo'd, zero-initialised) and the device context.
+subroutine allocate_main_module
+  use main_module
+  use rt6d_abi
+  implicit none
+  integer(c_int) :: n(6), up, down
+  allocate(E(1-onx:nx+onx, 1-onx:ny+onx, 1-onx:nz+onx, 1-onx:nk+onx, 1-onx:nl+onx, 1-onx:nm+onx))
+  allocate(dE, meanfl, mold=E)
+  allocate(Nsqr(1-onx:nx+onx, 1-onx:ny+onx, 1-onx:nz+onx))
+  allocate(U, V, mold=Nsqr)
+  allocate(xt(1-onx:nx+onx), xu(1-onx:nx+onx), dxt(1-onx:nx+onx), dxu(1-onx:nx+onx))
+  allocate(yt(1-onx:ny+onx), yu(1-onx:ny+onx), dyt(1-onx:ny+onx), dyu(1-onx:ny+onx))
+  allocate(zt(1-onx:nz+onx), zu(1-onx:nz+onx), dzt(1-onx:nz+onx), dzu(1-onx:nz+onx))
+  allocate(kt(1-onx:nk+onx), ku(1-onx:nk+onx), dkt(1-onx:nk+onx), dku(1-onx:nk+onx))
+  allocate(lt(1-onx:nl+onx), lu(1-onx:nl+onx), dlt(1-onx:nl+onx), dlu(1-onx:nl+onx))
+  allocate(mt(1-onx:nm+onx), mu(1-onx:nm+onx), dmt(1-onx:nm+onx), dmu(1-onx:nm+onx))
+  E = 0; dE = 0; meanfl = 0; Nsqr = 0; U = 0; V = 0
+  xt = 0; xu = 0; dxt = 0; dxu = 0; yt = 0; yu = 0; dyt = 0; dyu = 0
+  zt = 0; zu = 0; dzt = 0; dzu = 0; kt = 0; ku = 0; dkt = 0; dku = 0
+  lt = 0; lu = 0; dlt = 0; dlu = 0; mt = 0; mu = 0; dmt = 0; dmu = 0
+  n = [nx, ny, nz, nk, nl, nm]
+  up = merge(1, 0, enable_upper_reflection)
+  down = merge(1, 0, enable_lower_reflection)
+  call rt6d_check(rt6d_create(n, up, down, int(step_precision, c_int), rt6d_handle), 'allocate_main_module')
+end subroutine allocate_main_module
+
+! u-centred grid from T spacings (calc_grid.f90:300-322 semantics): the U
+! points sit half way between T points; all arrays have n entries.
+subroutine u_centered_grid(dyt_, dyu_, yt_, yu_, n)
+  implicit none
+  integer, intent(in) :: n
+  real*8, intent(in) :: dyt_(n)
+  real*8, intent(out) :: dyu_(n), yt_(n), yu_(n)
+  integer :: i
+  yu_(1) = 0
+  do i = 2, n
+    yu_(i) = yu_(i-1) + dyt_(i)
+  enddo
+  yt_(1) = yu_(1) - dyt_(1)*0.5d0
+  do i = 2, n
+    yt_(i) = 2*yu_(i-1) - yt_(i-1)
+  enddo
+  do i = 1, n-1
+    dyu_(i) = yt_(i+1) - yt_(i)
+  enddo
+  dyu_(n) = 2*dyt_(n) - dyu_(max(1, n-1))
+end subroutine u_centered_grid
+
+! T/U positions and spacings of all six axes from the T spacings set by
+! set_grid (calc_grid.f90:104-177) and the monotonicity checks of :260-293.
+subroutine calc_grid
+  use main_module
+  implicit none
+  integer :: i
+  call axis_grid(dxt, dxu, xt, xu, nx, 1, 0d0)
+  call axis_grid(dyt, dyu, yt, yu, ny, 1, 0d0)
+  call axis_grid(dzt, dzu, zt, zu, nz, 2, 0d0)
+  call axis_grid(dkt, dku, kt, ku, nk, 3, k_fixed)
+  call axis_grid(dlt, dlu, lt, lu, nl, 3, l_fixed)
+  call axis_grid(dmt, dmu, mt, mu, nm, 3, m_fixed)
+  do i = 2, nz
+    if (dzu(i) - dzu(i-1) > 1d-12 .or. dzt(i) - dzt(i-1) > 1d-12) &
+      call halt_stop(' in calc_grid: dzu/dzt do not decrease monotonically')
+  enddo
+  do i = 2, nm/2
+    if (dmu(i) - dmu(i-1) > 1d-12 .or. dmt(i) - dmt(i-1) > 1d-12) &
+      call halt_stop(' in calc_grid: dmu/dmt do not decrease monotonically')
+  enddo
+  do i = 2, nl/2
+    if (dlu(i) - dlu(i-1) > 1d-12 .or. dlt(i) - dlt(i-1) > 1d-12) &
+      call halt_stop(' in calc_grid: dlu/dlt do not decrease monotonically')
+  enddo
+contains
+  ! kind 1: shift by du(1) (x, y); 2: u(n) = 0 (z); 3: u(n/2) = 0 or fixed (wavenumbers)
+  subroutine axis_grid(d, du, t, u, n, kind, fixed)
+    integer, intent(in) :: n, kind
+    real*8, intent(inout) :: d(1-onx:n+onx)
+    real*8, intent(out) :: du(1-onx:n+onx), t(1-onx:n+onx), u(1-onx:n+onx)
+    real*8, intent(in) :: fixed
+    real*8 :: sh
+    integer :: ii
+    do ii = 1, onx
+      d(n+ii) = d(n); d(1-ii) = d(1)
+    enddo
+    call u_centered_grid(d, du, t, u, n + 2*onx)
+    select case (kind)
+    case (1)
+      sh = du(1)
+    case (2)
+      sh = u(n)
+    case default
+      if (n == 1) then
+        t = fixed; u = fixed
+        return
+      endif
+      sh = u(n/2)
+    end select
+    t = t - sh
+    u = u - sh
+  end subroutine axis_grid
+end subroutine calc_grid
+
+! Host E -> device (after set_initial_conditions, a forcing or a restart).
+subroutine e_to_device
+  use main_module
+  use rt6d_abi
+  implicit none
+  call rt6d_check(rt6d_upload_E(rt6d_handle, E, RT6D_LAYOUT_HALO), 'e_to_device')
+  host_E_current = .true.
+end subroutine e_to_device
+
+! Device E -> host interior (ghost entries are left as they were).
+subroutine e_to_host
+  use main_module
+  use rt6d_abi
+  implicit none
+  if (host_E_current) return
+  call rt6d_check(rt6d_download(rt6d_handle, RT6D_FIELD_E, RT6D_LAYOUT_HALO, E), 'e_to_host')
+  host_E_current = .true.
+end subroutine e_to_host
+
+! calc_parameter.f90:2 on the device: grids, background and E go over, the
+! coefficient fields are computed there and stay there.
+subroutine calc_parameter
+  use main_module
+  use rt6d_abi
+  implicit none
+  type(c_ptr) :: h
+  h = rt6d_handle
+  call rt6d_check(rt6d_set_grid(h, 0_c_int, xt, dxt), 'calc_parameter/x grid')
+  call rt6d_check(rt6d_set_grid(h, 1_c_int, yt, dyt), 'calc_parameter/y grid')
+  call rt6d_check(rt6d_set_grid(h, 2_c_int, zt, dzt), 'calc_parameter/z grid')
+  call rt6d_check(rt6d_set_grid(h, 3_c_int, kt, dkt), 'calc_parameter/k grid')
+  call rt6d_check(rt6d_set_grid(h, 4_c_int, lt, dlt), 'calc_parameter/l grid')
+  call rt6d_check(rt6d_set_grid(h, 5_c_int, mt, dmt), 'calc_parameter/m grid')
+  call rt6d_check(rt6d_set_background(h, Nsqr, U, V, f0, sign_of_omega_i), 'calc_parameter/background')
+  call rt6d_check(rt6d_calc_parameter(h), 'calc_parameter')
+  call e_to_device
+end subroutine calc_parameter
+
+! integrate.f90:4 -- one forward-Euler step of E on the device.
+subroutine integrate
+  use main_module
+  use rt6d_abi
+  implicit none
+  call rt6d_check(rt6d_step(rt6d_handle, dt, 1_c_int), 'integrate')
+  host_E_current = .false.
+end subroutine integrate
+
+! superbee_fluxes.f90:4 -- face fluxes of the current E, kept on the device
+! (read them with rt6d_download).
+subroutine superbee_fluxes
+  use main_module
+  use rt6d_abi
+  implicit none
+  call rt6d_check(rt6d_fluxes(rt6d_handle, dt), 'superbee_fluxes')
+end subroutine superbee_fluxes
+
+! diagnose.f90:45-329 -- energy budget of the last step, printed as the
+! reference prints it (:316-329).
+subroutine diagnose
+  use main_module
+  use rt6d_abi
+  implicit none
+  type(rt6d_diag_t) :: d
+  real*8, allocatable :: Ei(:), Mi(:), Di(:)
+  real*8 :: b(12)
+  allocate(Ei(nx*ny*nz), Mi(nx*ny*nz), Di(6*nx*ny*nz))
+  call rt6d_check(rt6d_diagnose(rt6d_handle, dt, d, Ei, Mi, Di), 'diagnose')
+  b = d%bfl
+  print '(a,f15.2,a)', ' snapshot at ', itt*dt, ' s'
+  print '(a,6f8.2)', ' max CFL in x,y,z,k,l,m-dir = ', d%cfl
+  if (d%mfl_max > 0) print '(a,es12.4)', ' mean flow interaction scale ', dt/d%mfl_max
+  print '(a,es14.6)', ' total dE          : ', d%eint
+  print '(a,es14.6)', ' dE by mean-flow   : ', d%mflint
+  print '(a,2es14.6)', ' by flux at min/max x  : ', b(1), b(2)
+  print '(a,2es14.6)', ' by flux at min/max y  : ', b(3), b(4)
+  print '(a,2es14.6)', ' by flux at min/max k  : ', b(7), b(8)
+  print '(a,2es14.6)', ' by flux at min/max l  : ', b(9), b(10)
+  print '(a,2es14.6)', ' by flux at min/max m  : ', b(11), b(12)
+  print '(a,es14.6)', ' residual : ', d%residual
+  deallocate(Ei, Mi, Di)
+end subroutine diagnose

@@ -37,7 +37,9 @@ Q_HOMOGENEOUS, Q_DEVICE_BYTES, Q_CELLS, Q_STEPS, Q_BYTES_PER_CELL = 1, 2, 3, 4, 
 EXPORTS = ("rt6d_version", "rt6d_last_error", "rt6d_device_count", "rt6d_create", "rt6d_destroy",
            "rt6d_build_grid", "rt6d_set_grid", "rt6d_set_background", "rt6d_calc_parameter", "rt6d_cfl_dt",
            "rt6d_upload_E", "rt6d_step", "rt6d_step_timed", "rt6d_fluxes", "rt6d_download", "rt6d_diagnose",
-           "rt6d_set_option", "rt6d_query", "rt6d_synchronize", "rt6d_copy_bandwidth")
+           "rt6d_set_option", "rt6d_query", "rt6d_synchronize", "rt6d_copy_bandwidth",
+           "rt6d_ensemble_create", "rt6d_ensemble_destroy", "rt6d_ensemble_add", "rt6d_ensemble_remove",
+           "rt6d_ensemble_step", "rt6d_ensemble_set_schedule", "rt6d_ensemble_tune", "rt6d_ensemble_info")
 
 
 class Diag(C.Structure):
@@ -94,6 +96,14 @@ def load_library(path: str | None = None):
         "rt6d_query": ([vp, C.c_int, C.POINTER(C.c_longlong)], C.c_int),
         "rt6d_synchronize": ([vp], C.c_int),
         "rt6d_copy_bandwidth": ([C.c_longlong, C.c_int, dp], C.c_int),
+        "rt6d_ensemble_create": ([C.c_int, C.POINTER(vp)], C.c_int),
+        "rt6d_ensemble_destroy": ([vp], C.c_int),
+        "rt6d_ensemble_add": ([vp, vp, C.c_double, ip], C.c_int),
+        "rt6d_ensemble_remove": ([vp, C.c_int], C.c_int),
+        "rt6d_ensemble_step": ([vp, C.c_int, dp], C.c_int),
+        "rt6d_ensemble_set_schedule": ([vp, C.c_int, C.c_int], C.c_int),
+        "rt6d_ensemble_tune": ([vp, C.c_int, ip, ip], C.c_int),
+        "rt6d_ensemble_info": ([vp, ip, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -279,3 +289,70 @@ class Model:
 
     def synchronize(self):
         _check(self.lib, self.lib.rt6d_synchronize(self._h), "rt6d_synchronize")
+
+
+class Ensemble:
+    """Many independent integrations advanced together (rt6d_ensemble_*).
+
+    Members are Models set up as for a single run (calc_parameter done, E
+    uploaded).  While a member, a Model is stepped only through the ensemble;
+    download/diagnose work on it between ensemble steps.
+    """
+
+    def __init__(self, precision=FP64):
+        self.lib = load_library()
+        self.precision = precision
+        self._h = C.c_void_p()
+        _check(self.lib, self.lib.rt6d_ensemble_create(precision, C.byref(self._h)), "rt6d_ensemble_create")
+        self.members: dict[int, Model] = {}
+
+    def close(self):
+        if self._h:
+            self.lib.rt6d_ensemble_destroy(self._h)
+            self._h = C.c_void_p()
+            self.members.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add(self, model: Model, dt: float | None = None) -> int:
+        dt = model.dt if dt is None else dt
+        slot = C.c_int()
+        _check(self.lib, self.lib.rt6d_ensemble_add(self._h, model._h, dt, C.byref(slot)), "rt6d_ensemble_add")
+        self.members[slot.value] = model
+        return slot.value
+
+    def remove(self, slot: int) -> Model:
+        _check(self.lib, self.lib.rt6d_ensemble_remove(self._h, slot), "rt6d_ensemble_remove")
+        return self.members.pop(slot)
+
+    def step(self, nsteps: int = 1) -> float:
+        """Advance every member nsteps steps; returns the device time in ms."""
+        ms = C.c_double()
+        _check(self.lib, self.lib.rt6d_ensemble_step(self._h, nsteps, C.byref(ms)), "rt6d_ensemble_step")
+        return ms.value
+
+    def set_schedule(self, blocks_per_cu: int, order: int = 0):
+        _check(self.lib, self.lib.rt6d_ensemble_set_schedule(self._h, blocks_per_cu, order),
+               "rt6d_ensemble_set_schedule")
+
+    def tune(self, trial_steps: int = 4):
+        b, o = C.c_int(), C.c_int()
+        _check(self.lib, self.lib.rt6d_ensemble_tune(self._h, trial_steps, C.byref(b), C.byref(o)),
+               "rt6d_ensemble_tune")
+        return b.value, o.value
+
+    def info(self):
+        m, c, s = C.c_int(), C.c_longlong(), C.c_longlong()
+        _check(self.lib, self.lib.rt6d_ensemble_info(self._h, C.byref(m), C.byref(c), C.byref(s)),
+               "rt6d_ensemble_info")
+        return {"members": m.value, "cells": c.value, "steps": s.value}

@@ -1,0 +1,131 @@
+"""Ensembles (rt6d_ensemble_*) against members run one at a time.
+
+An ensemble runs the same column-kernel body as a single context's fast
+path, so every member's E after any sequence of ensemble steps -- graph
+replays of two steps, single launches for odd counts, members joining and
+leaving, schedule changes, the autotuner -- must equal, bit for bit, the E
+of the same case stepped the same number of times on its own.
+"""
+import numpy as np
+import pytest
+
+from raytracer_3d_amd import cases, rt6d
+
+pytestmark = pytest.mark.gpu
+
+
+def _cases():
+    return [
+        cases.packet_case(n=(8, 4, 6, 4, 4, 6), name="packet_even_nz"),           # homogeneous, ZC=2
+        cases.packet_case(n=(6, 3, 5, 4, 3, 5), U0=0.2, Nscale=1300.0, name="packet_odd_nz"),  # ZC=1
+        cases.random_case((6, 5, 4, 4, 3, 4), seed=21),                            # general (lambda_x,y, kdot)
+        cases.random_case((5, 4, 6, 3, 2, 4), seed=22, horizontal=False, reflect_up=False),
+    ]
+
+
+def _model(case, precision, path=0):
+    m = rt6d.Model.from_case(case, precision=precision, path=path)
+    m.calc_parameter()
+    m.dt = m.cfl_dt(0.5)
+    return m
+
+
+def _alone(case, precision, nsteps, path=0):
+    with _model(case, precision, path) as m:
+        m.integrate(nsteps)
+        return m.download("E")
+
+
+@pytest.mark.parametrize("precision", [rt6d.FP64, rt6d.FP32], ids=["fp64", "fp32"])
+def test_ensemble_equals_individual_runs(precision, gpu_available):
+    cs = _cases()
+    ms = [_model(c, precision, path=1 if i == 3 else 0) for i, c in enumerate(cs)]
+    assert ms[0].homogeneous and not ms[2].homogeneous and not ms[3].homogeneous
+    with rt6d.Ensemble(precision) as ens:
+        slots = [ens.add(m) for m in ms]
+        ens.step(3)        # one graph (2 steps) + one single launch
+        ens.step(4)        # two graph replays from odd parity
+        ens.step(1)
+        info = ens.info()
+        assert info["members"] == 4 and info["steps"] == 8
+        for c, m in zip(cs, ms):
+            E = m.download("E")
+            ref = _alone(c, precision, 8, path=1 if c is cs[3] else 0)
+            assert np.array_equal(E, ref), f"{c.name}: max diff {np.max(np.abs(E - ref))}"
+        # membership changes: member 1 leaves after 8 steps, a new one joins
+        gone = ens.remove(slots[1])
+        new_case = cases.random_case((4, 4, 5, 4, 2, 4), seed=23)
+        new = _model(new_case, precision)
+        ens.add(new)
+        ens.set_schedule(2, 1)
+        ens.step(5)
+        ens.set_schedule(8, 0)
+        ens.step(2)
+        for c, m, n in ((cs[0], ms[0], 15), (cs[2], ms[2], 15), (cs[3], ms[3], 15), (new_case, new, 7)):
+            E = m.download("E")
+            ref = _alone(c, precision, n, path=1 if c is cs[3] else 0)
+            assert np.array_equal(E, ref), f"{c.name}: max diff {np.max(np.abs(E - ref))}"
+        # the removed member is a normal context again and resumes where it stopped
+        gone.integrate(2)
+        assert np.array_equal(gone.download("E"), _alone(cs[1], precision, 10))
+    for m in ms + [new]:
+        m.close()
+
+
+def test_ensemble_tune_and_state(gpu_available):
+    cs = _cases()[:3]
+    ms = [_model(c, rt6d.FP64) for c in cs]
+    with rt6d.Ensemble(rt6d.FP64) as ens:
+        for m in ms:
+            ens.add(m)
+        bpc, order = ens.tune(trial_steps=2)
+        assert bpc in (2, 4, 8) and order in (0, 1)
+        steps = ens.info()["steps"]
+        assert steps == 6 * (2 + 2)
+        ens.step(1)
+        for c, m in zip(cs, ms):
+            assert np.array_equal(m.download("E"), _alone(c, rt6d.FP64, steps + 1))
+        # a member is stepped only through its ensemble
+        with pytest.raises(rt6d.RT6DError, match="ensemble member"):
+            ms[0].integrate(1)
+        with pytest.raises(rt6d.RT6DError, match="ensemble member"):
+            ms[0].upload_E(cs[0].E0)
+        # diagnose works between ensemble steps and matches an individual run
+        d = ms[1].diagnose()
+        with _model(cs[1], rt6d.FP64) as solo:
+            solo.integrate(steps + 1)
+            ds = solo.diagnose()
+        assert d["eint"] == ds["eint"] and d["residual"] == ds["residual"]
+    # destroying the ensemble hands members back
+    ms[0].integrate(1)
+    for m in ms:
+        m.close()
+
+
+def test_ensemble_rejects_bad_members(gpu_available):
+    c = _cases()[0]
+    m64 = _model(c, rt6d.FP64)
+    m32 = _model(c, rt6d.FP32)
+    mex = _model(c, rt6d.FP64)
+    mex.set_option(rt6d.OPT_EXACT, 1)
+    bare = rt6d.Model(c.n)
+    with rt6d.Ensemble(rt6d.FP64) as ens:
+        with pytest.raises(rt6d.RT6DError, match="precision"):
+            ens.add(m32)
+        with pytest.raises(rt6d.RT6DError, match="exact"):
+            ens.add(mex)
+        with pytest.raises(rt6d.RT6DError, match="calc_parameter"):
+            ens.add(bare, dt=1.0)
+        with pytest.raises(rt6d.RT6DError, match="dt"):
+            ens.add(m64, dt=-1.0)
+        s = ens.add(m64)
+        with pytest.raises(rt6d.RT6DError, match="already"):
+            ens.add(m64)
+        # destroying a member context takes it out of the ensemble
+        m64.close()
+        assert ens.info()["members"] == 0
+        with pytest.raises(rt6d.RT6DError, match="slot"):
+            ens.remove(s)
+        ens.step(2)   # empty ensemble: nothing to do
+    for m in (m32, mex, bare):
+        m.close()
