@@ -114,18 +114,26 @@ def cpu_baseline(case, dt, steps):
             "sample": "failed: " + (last_err or "")}
 
 
-def _traffic(workload, kernel):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary."""
+def _traffic(workload, tag):
+    """HBM bytes per launch of the step kernel from the committed rocprofv3
+    --pmc summary (FETCH_SIZE and WRITE_SIZE, scaled by the copy kernel)."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(p):
         return None
     try:
         with open(p) as f:
             t = json.load(f)
-        e = t.get(workload, {}).get(kernel)
+        e = t.get(workload, {}).get(tag)
         return e["bytes_per_launch"] if e else None
     except Exception:  # noqa: BLE001
         return None
+
+
+def _step_kernel(precision, homog, nz):
+    """Name of the kernel one default step launches (kernel option 0: the
+    column kernel with two z cells per thread when nz is even)."""
+    zc = 2 if nz % 2 == 0 else 1
+    return f"k_step_col<{precision}, false, {'true' if homog else 'false'}, {zc}>"
 
 
 def main():
@@ -182,8 +190,8 @@ def main():
         extras["fp64_percell"] = run(rt6d.FP64, 0, kernel=1)
         extras["fp64_general_percell"] = run(rt6d.FP64, 1, kernel=1)
 
-    kname = ("k_step_march<double, false, true>" if main_res["homogeneous"]
-             else "k_step_march<double, false, false>")
+    kname = _step_kernel("double", main_res["homogeneous"], case.n[2])
+    ttag = "fp64_auto" if main_res["homogeneous"] else "fp64_general"
     achieved = main_res["achieved_GBs"]
     roofline = {
         "bound": "hbm",
@@ -191,7 +199,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": _traffic(workload, kname),
+        "traffic": _traffic(workload, ttag),
         "kernel": kname,
         "algorithmic_bytes_per_cell": main_res["bytes_per_cell"],
         "launch_ms": round(main_res["launch_ms"], 4),

@@ -63,6 +63,14 @@ __device__ __forceinline__ void bst<float>(float v, __amdgpu_buffer_rsrc_t r, ui
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, (int)voff, (int)soff, 0);
 }
 
+// Wave-uniform read of a table that no kernel writes while it runs: through
+// the constant address space, so that a uniform index becomes a scalar load.
+template <typename T>
+__device__ __forceinline__ T sld(const T *p, uint32_t i)
+{
+    return ((const __attribute__((address_space(4))) T *)p)[i];
+}
+
 // The work of logical block b (256 threads) of one integration.
 template <typename T, bool EXACT, bool HOMOG, int ZC>
 __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, const ColCfg &cc, uint32_t b)
@@ -187,7 +195,7 @@ __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, con
     if (am) {
 #pragma unroll
         for (int j = 0; j < ZC; ++j) {
-            const T vm0 = HOMOG ? p.hom.vm[P0 + j] : bld<T>(rV[5], oc, poff(j, 1));
+            const T vm0 = HOMOG ? sld(p.hom.vm, P0 + j) : bld<T>(rV[5], oc, poff(j, 1));
             FLm[j] = vm0 * w0[j];   // m face 0: v*E(1)
         }
     }
@@ -214,9 +222,9 @@ __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, con
             for (int i = 0; i <= ZC; ++i) {
                 const int fz = z0 - 1 + i;
                 T v;
-                if (HOMOG) v = p.hom.vz[(uint32_t)fz + (uint32_t)(nz + 1) * q3];
+                if (HOMOG) v = sld(p.hom.vz, (uint32_t)fz + (uint32_t)(nz + 1) * q3);
                 else v = bld<T>(rV[2], oc, ((uint32_t)fz + (uint32_t)(nz + 1) * q3) * planeB);
-                Fz[i] = face_flux<T, EXACT>(v, dtv, p.ax.dx[2][fz], p.ax.cfl[2][fz], zs[i], zs[i + 1], zs[i + 2],
+                Fz[i] = face_flux<T, EXACT>(v, dtv, sld(p.ax.dx[2], fz), sld(p.ax.cfl[2], fz), zs[i], zs[i + 1], zs[i + 2],
                                             zs[i + 3]);
             }
         }
@@ -232,7 +240,7 @@ __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, con
                 const T a3 = bld<T>(rE, ox[2], pb), a4 = bld<T>(rE, ox[3], pb);
                 T vL, vR;
                 if (HOMOG) {
-                    const T v0 = p.hom.vx3[q2], vi = p.hom.vx3[p.hom.n4 + q2], vb = p.hom.vx3[2 * p.hom.n4 + q2];
+                    const T v0 = sld(p.hom.vx3, q2), vi = sld(p.hom.vx3, p.hom.n4 + q2), vb = sld(p.hom.vx3, 2 * p.hom.n4 + q2);
                     vL = x == 1 ? v0 : vi;
                     vR = x == nx ? vb : vi;
                 } else {
@@ -248,7 +256,7 @@ __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, con
                 const T a3 = bld<T>(rE, oy[2], pb), a4 = bld<T>(rE, oy[3], pb);
                 T vL, vR;
                 if (HOMOG) {
-                    const T v0 = p.hom.vy3[q2], vi = p.hom.vy3[p.hom.n4 + q2], vb = p.hom.vy3[2 * p.hom.n4 + q2];
+                    const T v0 = sld(p.hom.vy3, q2), vi = sld(p.hom.vy3, p.hom.n4 + q2), vb = sld(p.hom.vy3, 2 * p.hom.n4 + q2);
                     vL = y == 1 ? v0 : vi;
                     vR = y == ny ? vb : vi;
                 } else {
@@ -261,7 +269,7 @@ __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, con
             }
             if (az) {
                 const int zc = z0 + j;
-                dE = div_term<T, EXACT>(dE, Fz[j + 1], Fz[j], p.ax.dx[2][zc], p.ax.rdx[2][zc]);
+                dE = div_term<T, EXACT>(dE, Fz[j + 1], Fz[j], sld(p.ax.dx[2], zc), sld(p.ax.rdx[2], zc));
             }
             if (ak) {
                 const T km2 = bld<T>(rE, oc, pb + (uint32_t)(kpl[0] * (int)planeB));
@@ -288,13 +296,13 @@ __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, con
                 dE = div_term<T, EXACT>(dE, FR, FL, dlr, rll);
             }
             if (am) {
-                const T vR = HOMOG ? p.hom.vm[q2 + p.hom.inner4m] : bld<T>(rV[5], oc, pb + mB);
-                const double dm = p.ax.dx[5][m];
-                const T FR = face_flux<T, EXACT>(vR, dtv, dm, p.ax.cfl[5][m], wm1[j], e0, wp1[j], wp2[j]);
-                dE = div_term<T, EXACT>(dE, FR, FLm[j], dm, p.ax.rdx[5][m]);
+                const T vR = HOMOG ? sld(p.hom.vm, q2 + p.hom.inner4m) : bld<T>(rV[5], oc, pb + mB);
+                const double dm = sld(p.ax.dx[5], m);
+                const T FR = face_flux<T, EXACT>(vR, dtv, dm, sld(p.ax.cfl[5], m), wm1[j], e0, wp1[j], wp2[j]);
+                dE = div_term<T, EXACT>(dE, FR, FLm[j], dm, sld(p.ax.rdx[5], m));
                 FLm[j] = FR;
             }
-            const T lam = HOMOG ? p.hom.lam[q2] : bld<T>(rL, oc, pb);
+            const T lam = HOMOG ? sld(p.hom.lam, q2) : bld<T>(rL, oc, pb);
             const T mf = e0 * lam;
             dE = dE - mf;
             T enew;
