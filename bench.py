@@ -130,10 +130,14 @@ def _traffic(workload, tag):
 
 
 def _step_kernel(precision, homog, nz):
-    """Name of the kernel one default step launches (kernel option 0: the
-    column kernel with two z cells per thread when nz is even)."""
+    """Name of the kernel one default step launches at this workload (kernel
+    option 0: the column kernel, two z cells per thread when nz is even,
+    compiled for all advected axes active, y neighbours from LDS on the
+    general path)."""
     zc = 2 if nz % 2 == 0 else 1
-    return f"k_step_col<{precision}, false, {'true' if homog else 'false'}, {zc}>"
+    h = "true" if homog else "false"
+    ldsy = "false" if homog else "true"
+    return f"k_step_col<{precision}, false, {h}, {zc}, true, {ldsy}>"
 
 
 def main():

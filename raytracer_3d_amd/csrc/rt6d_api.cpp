@@ -952,12 +952,19 @@ int rt6d_set_option(rt6d_ctx *c, int option, int value)
         c->opt_block = value;
         return 0;
     case RT6D_OPT_KERNEL:
-        if (value < 0 || value > 6) return fail(RT6D_EINVAL, "kernel option %d", value);
+        if (value < 0 || value > 11) return fail(RT6D_EINVAL, "kernel option %d", value);
+        if (value == 7 || value == 8) {
+            const int zc = value == 8 ? 2 : 1;
+            const bool fits = c->prec == RT6D_FP64 ? plane_fits<double>(c->g, zc) : plane_fits<float>(c->g, zc);
+            if (!fits)
+                return fail(RT6D_EINVAL, "plane kernel needs nx*ny <= 1024, nz divisible by %d and its LDS "
+                            "buffers within 64 KiB (have nx*ny = %d)", zc, c->n[0] * c->n[1]);
+        }
         if ((value == 2 || value == 3) && !march_fits(c->g))
             return fail(RT6D_EINVAL, "marching kernel needs nx*ny <= 1024 (have %d)", c->n[0] * c->n[1]);
-        if ((value == 3 || value == 5) && c->n[2] % 2 != 0)
+        if ((value == 3 || value == 5 || value == 10) && c->n[2] % 2 != 0)
             return fail(RT6D_EINVAL, "two z cells per thread need an even nz (have %d)", c->n[2]);
-        if (value == 6 && c->n[2] % 4 != 0)
+        if ((value == 6 || value == 11) && c->n[2] % 4 != 0)
             return fail(RT6D_EINVAL, "four z cells per thread need nz divisible by 4 (have %d)", c->n[2]);
         c->opt_kernel = value;
         return 0;
@@ -1039,7 +1046,8 @@ int ens_rebuild_t(rt6d_ensemble *e)
             const int a = c->cur;
             M.p[0] = step_ptrs<T>(c, c->d_E[a], c->d_E[a ^ 1], e->dt[i]);
             M.p[1] = step_ptrs<T>(c, c->d_E[a ^ 1], c->d_E[a], e->dt[i]);
-            M.variant = (c->homog ? 1u : 0u) | (zc == 2 ? 2u : 0u);
+            const bool full = c->homog ? is_full<true>(c->g) : is_full<false>(c->g);
+            M.variant = (c->homog ? 1u : 0u) | (zc == 2 ? 2u : 0u) | (full ? 4u : 0u);
             e->cur0[i] = a;
             nbl.push_back(col_blocks<T>(c->g, M.cc));
         } else {

@@ -8,8 +8,12 @@
 //    m is carried as the left face of step m+1;
 //  * the ZC cells share their ZC-1 inner z faces; the four outer z
 //    neighbours are read one plane ahead;
-//  * x and y neighbours are plain loads (L1/L2 hits: the neighbouring lanes
-//    read the same lines); both faces are computed per cell;
+//  * x neighbours come from the neighbouring lanes (lane = x + nx*y within
+//    the wavefront) by DPP lane shifts; only the two lanes at either end of
+//    a wavefront load theirs.  y neighbours are plain loads (L1/L2 hits:
+//    the neighbouring lanes read the same lines).  Both faces are computed
+//    per cell.  The vector-memory pipeline, not HBM or VALU, bounds this
+//    kernel, so every load taken off it counts;
 //  * the (x,y) index is padded to a multiple of 64 so that a wavefront never
 //    spans two (z,k,l) planes.  Every access is then a raw buffer access with
 //    a wave-uniform byte offset (scalar register) and a per-thread 32-bit
@@ -71,8 +75,41 @@ __device__ __forceinline__ T sld(const T *p, uint32_t i)
     return ((const __attribute__((address_space(4))) T *)p)[i];
 }
 
+// Whole-wavefront lane shifts (DPP wave_shr:1 / wave_shl:1): lane i receives
+// v of lane i-1 (shr) or i+1 (shl); lanes 0 and 63 keep their own v.
+__device__ __forceinline__ int dpp_shr1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_shl1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false); }
+template <bool SHR>
+__device__ __forceinline__ double lane_shift(double v)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const int lo = (int)(uint32_t)u, hi = (int)(uint32_t)(u >> 32);
+    const int rl = SHR ? dpp_shr1(lo) : dpp_shl1(lo), rh = SHR ? dpp_shr1(hi) : dpp_shl1(hi);
+    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)rh << 32) | (uint32_t)rl);
+}
+template <bool SHR>
+__device__ __forceinline__ float lane_shift(float v)
+{
+    const int i = __builtin_bit_cast(int, v);
+    return __builtin_bit_cast(float, SHR ? dpp_shr1(i) : dpp_shl1(i));
+}
+
+// Axis mask of the common case in which every axis that the path advects is
+// active (x, y, z, m on the homogeneous path; all six on the general path).
+// With FULL the kernel is compiled for that mask, without branches on it.
+template <bool HOMOG>
+constexpr unsigned full_mask() { return HOMOG ? 1u | 2u | 4u | 32u : 63u; }
+
+template <bool HOMOG>
+inline bool is_full(const Geo &g) { return (g.act & full_mask<HOMOG>()) == full_mask<HOMOG>() && (HOMOG || g.act == 63); }
+
 // The work of logical block b (256 threads) of one integration.
-template <typename T, bool EXACT, bool HOMOG, int ZC>
+// LDSY: the y neighbours come from the block's copy of its (x,y) planes in
+// LDS (one barrier per m step, two buffers alternating with m) where the
+// neighbour lies inside the block's 256 positions, else from memory.  Lanes
+// beyond the grid then stay to the end (for the barriers) on clamped
+// in-bounds coordinates and store nothing.
+template <typename T, bool EXACT, bool HOMOG, int ZC, bool FULL = false, bool LDSY = false>
 __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, const ColCfg &cc, uint32_t b)
 {
     const uint32_t t = b * blockDim.x + threadIdx.x;
@@ -80,18 +117,25 @@ __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, con
     const int nxy = nx * ny;
     // the plane part of the thread index is uniform over the wavefront
     const uint32_t pl = __builtin_amdgcn_readfirstlane(fdiv(t, cc.dxyp));
-    const int xy = (int)(t - pl * (uint32_t)cc.nxyp);
+    int xy = (int)(t - pl * (uint32_t)cc.nxyp);
     const uint32_t kl = fdiv(pl, cc.dzb);
-    const int zb = (int)(pl - kl * (uint32_t)cc.nzb);
-    const int bl = (int)fdiv(kl, g.div[3]);
-    const int bk = (int)kl - bl * nk;
-    if (bl >= nl) return;   // whole wavefronts beyond the grid
-    if (xy >= nxy) return;  // padding lanes (no barriers below)
+    int zb = (int)(pl - kl * (uint32_t)cc.nzb);
+    int bl = (int)fdiv(kl, g.div[3]);
+    int bk = (int)kl - bl * nk;
+    bool live = true;
+    if constexpr (LDSY) {
+        if (bl >= nl) { bl = nl - 1; bk = nk - 1; zb = cc.nzb - 1; live = false; }
+        if (xy >= nxy) { xy = nxy - 1; live = false; }
+    } else {
+        if (bl >= nl) return;   // whole wavefronts beyond the grid
+        if (xy >= nxy) return;  // padding lanes (no barriers below)
+    }
     const int y0 = (int)fdiv((uint32_t)xy, g.div[0]);
     const int x0 = xy - y0 * nx;
     const int x = x0 + 1, y = y0 + 1;
     const int z0 = zb * ZC + 1, k = bk + 1, l = bl + 1;
     constexpr uint32_t S = sizeof(T);
+    const uint32_t lane = threadIdx.x & 63u;   // = xy mod 64: planes start on wavefront boundaries
 
     const size_t s5 = (size_t)g.inner[5];
     const uint32_t nzkl = (uint32_t)nz * nk * nl;
@@ -103,13 +147,16 @@ __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, con
     const uint32_t planeB = (uint32_t)nxy * S;          // bytes per (x,y) plane
     const uint32_t mB = (uint32_t)(s5 * S);             // bytes per m step
     const uint32_t base0 = P0 * planeB;                 // plane (z0,k,l,m=1)
-    const bool ax = g.act & 1, ay = g.act & 2, az = g.act & 4;
-    const bool ak = !HOMOG && (g.act & 8), al = !HOMOG && (g.act & 16), am = g.act & 32;
+    const bool ax = FULL || (g.act & 1), ay = FULL || (g.act & 2), az = FULL || (g.act & 4);
+    const bool ak = !HOMOG && (FULL || (g.act & 8)), al = !HOMOG && (FULL || (g.act & 16));
+    const bool am = FULL || (g.act & 32);
     const double dtv = p.dt;
 
     // per-thread byte offsets inside an (x,y) plane, fixed for the march
     const uint32_t oc = (uint32_t)xy * S;
     uint32_t ox[4], oy[4];
+    int ly[4];      // LDSY: block-local index of the y neighbour (own index when outside the block)
+    bool yin[4];
     {
         const int xs[4] = {x - 2 < 1 ? 1 : x - 2, x - 1 < 1 ? 1 : x - 1, x + 1 > nx ? nx : x + 1,
                            x + 2 > nx ? nx : x + 2};
@@ -119,6 +166,9 @@ __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, con
         for (int q = 0; q < 4; ++q) {
             ox[q] = (uint32_t)((xs[q] - 1) + nx * y0) * S;
             oy[q] = (uint32_t)(x0 + nx * (ys[q] - 1)) * S;
+            const int li = (int)threadIdx.x + (ys[q] - y) * nx;
+            yin[q] = li >= 0 && li < (int)blockDim.x;
+            ly[q] = yin[q] ? li : (int)threadIdx.x;
         }
     }
     double cxl = 0, cxr = 0, dxl = 1, dxr = 1, rxx = 0, cyl = 0, cyr = 0, dyl = 1, dyr = 1, ryy = 0;
@@ -209,6 +259,14 @@ __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, con
 #pragma unroll
         for (int q = 0; q < 4; ++q) znx[q] = (az && m < nm) ? ldz(q, m + 1) : T(0);
 
+        T *sb = nullptr;
+        if constexpr (LDSY) {
+            __shared__ T sEy[2][ZC][256];
+            sb = &sEy[m & 1][0][0];
+#pragma unroll
+            for (int j = 0; j < ZC; ++j) sb[j * 256 + threadIdx.x] = w0[j];
+            __syncthreads();
+        }
         T Fz[ZC + 1];
         if (az) {
             T zs[ZC + 4];
@@ -236,8 +294,20 @@ __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, con
             const T e0 = w0[j];
             T dE = T(0);
             if (ax) {
-                const T a0 = bld<T>(rE, ox[0], pb), a1 = bld<T>(rE, ox[1], pb);
-                const T a3 = bld<T>(rE, ox[2], pb), a4 = bld<T>(rE, ox[3], pb);
+                // E(x-2), E(x-1), E(x+1), E(x+2) of the same plane: the lanes' own e0
+                T a1 = lane_shift<true>(e0), a3 = lane_shift<false>(e0);
+                T a0 = lane_shift<true>(a1), a4 = lane_shift<false>(a3);
+                if (lane < 2 || lane >= 62) {   // neighbours held by the adjacent wavefronts
+                    if (lane == 0) a1 = bld<T>(rE, ox[1], pb);
+                    if (lane < 2) a0 = bld<T>(rE, ox[0], pb);
+                    if (lane == 63) a3 = bld<T>(rE, ox[2], pb);
+                    if (lane >= 62) a4 = bld<T>(rE, ox[3], pb);
+                }
+                // zero-gradient ghosts at x = 1 and x = nx
+                a1 = x == 1 ? e0 : a1;
+                a0 = x <= 2 ? a1 : a0;
+                a3 = x == nx ? e0 : a3;
+                a4 = x >= nx - 1 ? a3 : a4;
                 T vL, vR;
                 if (HOMOG) {
                     const T v0 = sld(p.hom.vx3, q2), vi = sld(p.hom.vx3, p.hom.n4 + q2), vb = sld(p.hom.vx3, 2 * p.hom.n4 + q2);
@@ -252,8 +322,25 @@ __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, con
                 dE = div_term<T, EXACT>(dE, FR, FL, dxr, rxx);
             }
             if (ay) {
-                const T a0 = bld<T>(rE, oy[0], pb), a1 = bld<T>(rE, oy[1], pb);
-                const T a3 = bld<T>(rE, oy[2], pb), a4 = bld<T>(rE, oy[3], pb);
+                T a0, a1, a3, a4;
+                if constexpr (LDSY) {
+                    const T *sj = sb + j * 256;
+                    a0 = sj[ly[0]];
+                    a1 = sj[ly[1]];
+                    a3 = sj[ly[2]];
+                    a4 = sj[ly[3]];
+                    if (!(yin[0] && yin[1] && yin[2] && yin[3])) {   // rows next to the block's ends
+                        if (!yin[0]) a0 = bld<T>(rE, oy[0], pb);
+                        if (!yin[1]) a1 = bld<T>(rE, oy[1], pb);
+                        if (!yin[2]) a3 = bld<T>(rE, oy[2], pb);
+                        if (!yin[3]) a4 = bld<T>(rE, oy[3], pb);
+                    }
+                } else {
+                    a0 = bld<T>(rE, oy[0], pb);
+                    a1 = bld<T>(rE, oy[1], pb);
+                    a3 = bld<T>(rE, oy[2], pb);
+                    a4 = bld<T>(rE, oy[3], pb);
+                }
                 T vL, vR;
                 if (HOMOG) {
                     const T v0 = sld(p.hom.vy3, q2), vi = sld(p.hom.vy3, p.hom.n4 + q2), vb = sld(p.hom.vy3, 2 * p.hom.n4 + q2);
@@ -308,7 +395,7 @@ __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, con
             T enew;
             if constexpr (EXACT) enew = e0 + p.dt * dE;
             else enew = e0 + (T)p.dt * dE;
-            bst<T>(enew, rO, oc, pb);
+            if (!LDSY || live) bst<T>(enew, rO, oc, pb);
         }
 #pragma unroll
         for (int j = 0; j < ZC; ++j) {
@@ -322,10 +409,10 @@ __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, con
     }
 }
 
-template <typename T, bool EXACT, bool HOMOG, int ZC>
+template <typename T, bool EXACT, bool HOMOG, int ZC, bool FULL, bool LDSY = false>
 __global__ void __launch_bounds__(256) k_step_col(const Geo g, const StepPtrs<T> p, const ColCfg cc, uint32_t nbl)
 {
     const uint32_t b = xcd_logical_block(blockIdx.x, nbl);
-    if (b >= nbl) return;
-    col_body<T, EXACT, HOMOG, ZC>(g, p, cc, b);
+    if (b >= nbl) return;   // uniform over the block
+    col_body<T, EXACT, HOMOG, ZC, FULL, LDSY>(g, p, cc, b);
 }

@@ -119,23 +119,39 @@ __device__ __forceinline__ double sb_flux_exact(double v, double dt, double dxf,
     return v * (E1 + E0) * 0.5 - fabs(v) * ((1.0 - cr) + ucfl * cr) * Rj * 0.5;
 }
 
-// The same flux without a divide: phi(r)*R written with min/max of R and the
-// upwind difference (r = Ru/R), which is phi(r)*R exactly in real arithmetic
-// and 0 for R == 0 as in the reference (where the anti-diffusive term is
-// multiplied by Rj = 0).  F = 0.5*(v*(E1+E0) - |v|*(R - (1-c)*phiR)).
+// a with its sign flipped where b is negative (a * sign(b), exact)
+__device__ __forceinline__ double flip_sign_by(double a, double b)
+{
+    const uint64_t ua = __builtin_bit_cast(uint64_t, a), ub = __builtin_bit_cast(uint64_t, b);
+    return __builtin_bit_cast(double, ua ^ (ub & 0x8000000000000000ull));
+}
+__device__ __forceinline__ float flip_sign_by(float a, float b)
+{
+    const uint32_t ua = __builtin_bit_cast(uint32_t, a), ub = __builtin_bit_cast(uint32_t, b);
+    return __builtin_bit_cast(float, ua ^ (ub & 0x80000000u));
+}
+
+// The same flux without a divide.  With R = E1-E0, Ru the upwind difference
+// and r = Ru/R, superbee's phi(r)*R = sign(R) * max(0, min(2*min(|R|,su),
+// max(|R|,su))) with su = Ru*sign(R): min/max/doubling only, so phi(r)*R is
+// exact (one of 0, R, Ru, 2R, 2Ru) and 0 for R == 0 as in the reference
+// (where the anti-diffusive term is multiplied by Rj = 0).
+// F = v/2*(E1+E0) - |v|/2*(R - (1-c)*phiR), c = |v|*dt/dx.
 template <typename T>
 __device__ __forceinline__ T sb_flux_fast(T v, T cfl_coef, T Em1, T E0, T E1, T E2)
 {
-    T R = E1 - E0;
-    T Ru = v > T(0) ? (E0 - Em1) : (E2 - E1);
-    T aR = fabs(R);
-    T su = R < T(0) ? -Ru : Ru;
-    T p = fmax(fmin(aR, su + su), fmin(aR + aR, su));
-    p = fmax(p, T(0));
-    T phiR = R < T(0) ? -p : p;
-    T av = fabs(v);
-    T c = av * cfl_coef;
-    return T(0.5) * (v * (E1 + E0) - av * (R - (T(1) - c) * phiR));
+    const T R = E1 - E0;
+    const T Ru = v > T(0) ? (E0 - Em1) : (E2 - E1);
+    const T su = flip_sign_by(Ru, R);
+    const T aR = fabs(R);
+    const T mn = fmin(aR, su), mx = fmax(aR, su);
+    T p;
+    if constexpr (sizeof(T) == 4) p = __builtin_amdgcn_fmed3f(mn + mn, T(0), mx);   // mx >= 0
+    else p = fmax(fmin(mn + mn, mx), T(0));
+    const T phiR = __builtin_copysign(p, R);
+    const T hv = T(0.5) * v;
+    const T c = fabs(v) * cfl_coef;
+    return hv * (E1 + E0) - fabs(hv) * (R - (T(1) - c) * phiR);
 }
 
 }  // namespace rt6d

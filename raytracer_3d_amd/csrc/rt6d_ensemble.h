@@ -14,7 +14,8 @@ struct EnsMember {
     Geo g;
     StepPtrs<T> p[2];   // p[0]: E[a] -> E[b]; p[1]: E[b] -> E[a]
     ColCfg cc;
-    uint32_t variant;   // bit 0: horizontally homogeneous tables; bit 1: two z cells per thread
+    uint32_t variant;   // bit 0: horizontally homogeneous tables; bit 1: two z cells per thread;
+                        // bit 2: every advected axis active
     uint32_t pad;
 };
 
@@ -36,10 +37,14 @@ __global__ void __launch_bounds__(256) k_ens_step(const EnsMember<T> *__restrict
         const EnsMember<T> &M = mem[tk.member];
         const StepPtrs<T> &P = M.p[parity & 1u];
         switch (M.variant) {
-        case 0: col_body<T, false, false, 1>(M.g, P, M.cc, tk.block); break;
-        case 1: col_body<T, false, true, 1>(M.g, P, M.cc, tk.block); break;
-        case 2: col_body<T, false, false, 2>(M.g, P, M.cc, tk.block); break;
-        default: col_body<T, false, true, 2>(M.g, P, M.cc, tk.block); break;
+        case 0: col_body<T, false, false, 1, false>(M.g, P, M.cc, tk.block); break;
+        case 1: col_body<T, false, true, 1, false>(M.g, P, M.cc, tk.block); break;
+        case 2: col_body<T, false, false, 2, false>(M.g, P, M.cc, tk.block); break;
+        case 3: col_body<T, false, true, 2, false>(M.g, P, M.cc, tk.block); break;
+        case 4: col_body<T, false, false, 1, true>(M.g, P, M.cc, tk.block); break;
+        case 5: col_body<T, false, true, 1, true>(M.g, P, M.cc, tk.block); break;
+        case 6: col_body<T, false, false, 2, true>(M.g, P, M.cc, tk.block); break;
+        default: col_body<T, false, true, 2, true>(M.g, P, M.cc, tk.block); break;
         }
     }
 }

@@ -166,6 +166,7 @@ __device__ __forceinline__ T div_term(T dE, T FR, T FL, double dxc, double rdxc)
 
 #include "rt6d_column.h"
 #include "rt6d_march.h"
+#include "rt6d_plane.h"
 #include "rt6d_ensemble.h"
 
 // Tendency fields of one step (the by-products integrate leaves behind:
@@ -423,14 +424,49 @@ hipError_t launch_march(const Geo &g, const StepPtrs<T> &p, bool homog, hipStrea
     const uint32_t nbl = (uint32_t)mc.nzb * g.n[3] * g.n[4];
     const uint32_t nbh = ((nbl + 7u) / 8u) * 8u;
     const size_t smem = 2 * ZC * (size_t)nthr * sizeof(T);
-    if (homog)
-        hipLaunchKernelGGL((k_step_march<T, EXACT, true, ZC>), dim3(nbh), dim3(nthr), smem, stream, g, p, mc, nbl);
-    else
-        hipLaunchKernelGGL((k_step_march<T, EXACT, false, ZC>), dim3(nbh), dim3(nthr), smem, stream, g, p, mc, nbl);
+    if (homog) {
+        if (is_full<true>(g))
+            hipLaunchKernelGGL((k_step_march<T, EXACT, true, ZC, true>), dim3(nbh), dim3(nthr), smem, stream, g, p, mc, nbl);
+        else
+            hipLaunchKernelGGL((k_step_march<T, EXACT, true, ZC, false>), dim3(nbh), dim3(nthr), smem, stream, g, p, mc, nbl);
+    } else {
+        if (is_full<false>(g))
+            hipLaunchKernelGGL((k_step_march<T, EXACT, false, ZC, true>), dim3(nbh), dim3(nthr), smem, stream, g, p, mc, nbl);
+        else
+            hipLaunchKernelGGL((k_step_march<T, EXACT, false, ZC, false>), dim3(nbh), dim3(nthr), smem, stream, g, p, mc, nbl);
+    }
     return hipGetLastError();
 }
 
+// The plane kernel keeps one thread per (x,y) column in one block and two
+// LDS buffers of x and y face fluxes for ZC planes (at most 64 KiB).
+template <typename T>
+inline bool plane_fits(const Geo &g, int zc)
+{
+    const uint32_t nxy = (uint32_t)g.n[0] * g.n[1];
+    const uint32_t nthr = ((nxy + 63u) / 64u) * 64u;
+    return nxy <= 1024 && g.n[2] % zc == 0 && 4ull * zc * nthr * sizeof(T) <= 65536ull;
+}
+
 template <typename T, bool EXACT, int ZC>
+hipError_t launch_plane(const Geo &g, const StepPtrs<T> &p, bool homog, hipStream_t stream)
+{
+    const uint32_t nxy = (uint32_t)g.n[0] * g.n[1];
+    const uint32_t nthr = ((nxy + 63u) / 64u) * 64u;
+    PlaneCfg pc;
+    pc.nzb = g.n[2] / ZC;
+    pc.dzb = make_fastdiv((uint32_t)pc.nzb);
+    const uint32_t nbl = (uint32_t)pc.nzb * g.n[3] * g.n[4];
+    const uint32_t nbh = ((nbl + 7u) / 8u) * 8u;
+    const size_t smem = 4 * ZC * (size_t)nthr * sizeof(T);
+    if (homog)
+        hipLaunchKernelGGL((k_step_plane<T, EXACT, true, ZC>), dim3(nbh), dim3(nthr), smem, stream, g, p, pc, nbl);
+    else
+        hipLaunchKernelGGL((k_step_plane<T, EXACT, false, ZC>), dim3(nbh), dim3(nthr), smem, stream, g, p, pc, nbl);
+    return hipGetLastError();
+}
+
+template <typename T, bool EXACT, int ZC, bool LDSY = false>
 hipError_t launch_col(const Geo &g, const StepPtrs<T> &p, bool homog, hipStream_t stream)
 {
     const uint32_t nxy = (uint32_t)g.n[0] * g.n[1];
@@ -442,10 +478,17 @@ hipError_t launch_col(const Geo &g, const StepPtrs<T> &p, bool homog, hipStream_
     const uint64_t nthreads = (uint64_t)cc.nxyp * cc.nzb * g.n[3] * g.n[4];
     const uint32_t nbl = (uint32_t)((nthreads + 255) / 256);
     const uint32_t nbh = ((nbl + 7u) / 8u) * 8u;
-    if (homog)
-        hipLaunchKernelGGL((k_step_col<T, EXACT, true, ZC>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
-    else
-        hipLaunchKernelGGL((k_step_col<T, EXACT, false, ZC>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
+    if (homog) {
+        if (is_full<true>(g))
+            hipLaunchKernelGGL((k_step_col<T, EXACT, true, ZC, true, LDSY>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
+        else
+            hipLaunchKernelGGL((k_step_col<T, EXACT, true, ZC, false, LDSY>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
+    } else {
+        if (is_full<false>(g))
+            hipLaunchKernelGGL((k_step_col<T, EXACT, false, ZC, true, LDSY>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
+        else
+            hipLaunchKernelGGL((k_step_col<T, EXACT, false, ZC, false, LDSY>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
+    }
     return hipGetLastError();
 }
 
@@ -464,12 +507,29 @@ inline int col_zc(const Geo &g, int kernel)
 template <typename T, bool EXACT>
 hipError_t launch_step_impl(const Geo &g, const StepPtrs<T> &p, bool homog, const LaunchCfg &lc)
 {
+    if (lc.kernel == 7 || lc.kernel == 8) {
+        const bool col_ok = (uint64_t)g.ncells * 2u * sizeof(T) < (1ull << 32);
+        if (col_ok && lc.kernel == 8 && plane_fits<T>(g, 2)) return launch_plane<T, EXACT, 2>(g, p, homog, lc.stream);
+        if (col_ok && plane_fits<T>(g, 1)) return launch_plane<T, EXACT, 1>(g, p, homog, lc.stream);
+    }
     if ((lc.kernel == 2 || lc.kernel == 3) && march_fits(g)) {
         if (march_zc(g, lc.kernel) == 2) return launch_march<T, EXACT, 2>(g, p, homog, lc.stream);
         return launch_march<T, EXACT, 1>(g, p, homog, lc.stream);
     }
     // the column kernel addresses each array with 32-bit byte offsets
     const bool col_fits = (uint64_t)g.ncells * 2u * sizeof(T) < (1ull << 32);
+    if (lc.kernel >= 9 && lc.kernel <= 11 && col_fits) {
+        const int zc = lc.kernel == 9 ? 1 : (lc.kernel == 10 ? 2 : 4);
+        if (zc == 4 && g.n[2] % 4 == 0) return launch_col<T, EXACT, 4, true>(g, p, homog, lc.stream);
+        if (zc >= 2 && g.n[2] % 2 == 0) return launch_col<T, EXACT, 2, true>(g, p, homog, lc.stream);
+        return launch_col<T, EXACT, 1, true>(g, p, homog, lc.stream);
+    }
+    if (lc.kernel == 0 && !homog && col_fits) {
+        // general path: the y neighbours from LDS (measured 15% faster at
+        // 24^3 x 16^3 in binary64; the homogeneous path is faster without)
+        if (g.n[2] % 2 == 0) return launch_col<T, EXACT, 2, true>(g, p, homog, lc.stream);
+        return launch_col<T, EXACT, 1, true>(g, p, homog, lc.stream);
+    }
     if (lc.kernel != 1 && col_fits) {
         const int zc = col_zc(g, lc.kernel);
         if (zc == 4) return launch_col<T, EXACT, 4>(g, p, homog, lc.stream);

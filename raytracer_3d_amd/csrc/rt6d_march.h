@@ -21,7 +21,7 @@ struct MarchCfg {
     int nzb;       // nz / ZC
 };
 
-template <typename T, bool EXACT, bool HOMOG, int ZC>
+template <typename T, bool EXACT, bool HOMOG, int ZC, bool FULL>
 __global__ void __launch_bounds__(1024) k_step_march(const Geo g, const StepPtrs<T> p, const MarchCfg mc,
                                                      uint32_t nb_logical)
 {
@@ -57,8 +57,9 @@ __global__ void __launch_bounds__(1024) k_step_march(const Geo g, const StepPtrs
     const uint32_t planeB = (uint32_t)nxy * S;
     const uint32_t mB = s5 * S;
     const uint32_t base0 = P0 * planeB;
-    const bool ax = g.act & 1, ay = g.act & 2, az = g.act & 4;
-    const bool ak = !HOMOG && (g.act & 8), al = !HOMOG && (g.act & 16), am = g.act & 32;
+    const bool ax = FULL || (g.act & 1), ay = FULL || (g.act & 2), az = FULL || (g.act & 4);
+    const bool ak = !HOMOG && (FULL || (g.act & 8)), al = !HOMOG && (FULL || (g.act & 16));
+    const bool am = FULL || (g.act & 32);
     const double dtv = p.dt;
     const uint32_t oc = (uint32_t)xy * S;
 
@@ -145,7 +146,7 @@ __global__ void __launch_bounds__(1024) k_step_march(const Geo g, const StepPtrs
     if (am) {
 #pragma unroll
         for (int j = 0; j < ZC; ++j) {
-            const T vm0 = HOMOG ? p.hom.vm[P0 + j] : bld<T>(rV[5], oc, poff(j, 1));
+            const T vm0 = HOMOG ? sld(p.hom.vm, P0 + j) : bld<T>(rV[5], oc, poff(j, 1));
             FLm[j] = vm0 * w0[j];
         }
     }
@@ -177,9 +178,9 @@ __global__ void __launch_bounds__(1024) k_step_march(const Geo g, const StepPtrs
             for (int i = 0; i <= ZC; ++i) {
                 const int fz = z0 - 1 + i;
                 T v;
-                if (HOMOG) v = p.hom.vz[(uint32_t)fz + (uint32_t)(nz + 1) * q3];
+                if (HOMOG) v = sld(p.hom.vz, (uint32_t)fz + (uint32_t)(nz + 1) * q3);
                 else v = bld<T>(rV[2], oc, ((uint32_t)fz + (uint32_t)(nz + 1) * q3) * planeB);
-                Fz[i] = face_flux<T, EXACT>(v, dtv, p.ax.dx[2][fz], p.ax.cfl[2][fz], zs[i], zs[i + 1], zs[i + 2],
+                Fz[i] = face_flux<T, EXACT>(v, dtv, sld(p.ax.dx[2], fz), sld(p.ax.cfl[2], fz), zs[i], zs[i + 1], zs[i + 2],
                                             zs[i + 3]);
             }
         }
@@ -195,7 +196,7 @@ __global__ void __launch_bounds__(1024) k_step_march(const Geo g, const StepPtrs
                 const T a0 = sj[ixs[0]], a1 = sj[ixs[1]], a3 = sj[ixs[2]], a4 = sj[ixs[3]];
                 T vL, vR;
                 if (HOMOG) {
-                    const T v0 = p.hom.vx3[q2], vi = p.hom.vx3[p.hom.n4 + q2], vb = p.hom.vx3[2 * p.hom.n4 + q2];
+                    const T v0 = sld(p.hom.vx3, q2), vi = sld(p.hom.vx3, p.hom.n4 + q2), vb = sld(p.hom.vx3, 2 * p.hom.n4 + q2);
                     vL = x == 1 ? v0 : vi;
                     vR = x == nx ? vb : vi;
                 } else {
@@ -210,7 +211,7 @@ __global__ void __launch_bounds__(1024) k_step_march(const Geo g, const StepPtrs
                 const T a0 = sj[iys[0]], a1 = sj[iys[1]], a3 = sj[iys[2]], a4 = sj[iys[3]];
                 T vL, vR;
                 if (HOMOG) {
-                    const T v0 = p.hom.vy3[q2], vi = p.hom.vy3[p.hom.n4 + q2], vb = p.hom.vy3[2 * p.hom.n4 + q2];
+                    const T v0 = sld(p.hom.vy3, q2), vi = sld(p.hom.vy3, p.hom.n4 + q2), vb = sld(p.hom.vy3, 2 * p.hom.n4 + q2);
                     vL = y == 1 ? v0 : vi;
                     vR = y == ny ? vb : vi;
                 } else {
@@ -223,7 +224,7 @@ __global__ void __launch_bounds__(1024) k_step_march(const Geo g, const StepPtrs
             }
             if (az) {
                 const int zc = z0 + j;
-                dE = div_term<T, EXACT>(dE, Fz[j + 1], Fz[j], p.ax.dx[2][zc], p.ax.rdx[2][zc]);
+                dE = div_term<T, EXACT>(dE, Fz[j + 1], Fz[j], sld(p.ax.dx[2], zc), sld(p.ax.rdx[2], zc));
             }
             if (ak) {
                 const T km2 = bld<T>(rE, oc, pb + (uint32_t)(kpl[0] * (int)planeB));
@@ -248,13 +249,13 @@ __global__ void __launch_bounds__(1024) k_step_march(const Geo g, const StepPtrs
                 dE = div_term<T, EXACT>(dE, FR, FL, dlr, rll);
             }
             if (am) {
-                const T vR = HOMOG ? p.hom.vm[q2 + p.hom.inner4m] : bld<T>(rV[5], oc, pb + mB);
-                const double dm = p.ax.dx[5][m];
-                const T FR = face_flux<T, EXACT>(vR, dtv, dm, p.ax.cfl[5][m], wm1[j], e0, wp1[j], wp2[j]);
-                dE = div_term<T, EXACT>(dE, FR, FLm[j], dm, p.ax.rdx[5][m]);
+                const T vR = HOMOG ? sld(p.hom.vm, q2 + p.hom.inner4m) : bld<T>(rV[5], oc, pb + mB);
+                const double dm = sld(p.ax.dx[5], m);
+                const T FR = face_flux<T, EXACT>(vR, dtv, dm, sld(p.ax.cfl[5], m), wm1[j], e0, wp1[j], wp2[j]);
+                dE = div_term<T, EXACT>(dE, FR, FLm[j], dm, sld(p.ax.rdx[5], m));
                 FLm[j] = FR;
             }
-            const T lam = HOMOG ? p.hom.lam[q2] : bld<T>(rL, oc, pb);
+            const T lam = HOMOG ? sld(p.hom.lam, q2) : bld<T>(rL, oc, pb);
             const T mf = e0 * lam;
             dE = dE - mf;
             T enew;
