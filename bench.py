@@ -148,6 +148,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=300)
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="arrival window of the serving extra (the full scenario is 120 s; 0 skips it)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -192,7 +195,14 @@ def main():
         extras["fp32_general"] = run(rt6d.FP32, 1)
         _log("binary64 step, per-cell kernel")
         extras["fp64_percell"] = run(rt6d.FP64, 0, kernel=1)
-        extras["fp64_general_percell"] = run(rt6d.FP64, 1, kernel=1)
+        from raytracer_3d_amd import serve
+        for prec, tag in ((rt6d.FP64, "ensemble96_fp64"), (rt6d.FP32, "ensemble96_fp32")):
+            _log(f"batched ensemble, 96 members x {args.ensemble_steps} steps ({tag[-4:]})")
+            extras[tag] = serve.ensemble_benchmark(96, args.ensemble_steps, prec, alone_members=24, log=_log)
+        if args.serve_seconds > 0:
+            _log(f"serving: Poisson 40 req/s for {args.serve_seconds:g} s, 200 GB budget")
+            extras["serving"] = serve.serve_benchmark(40.0, args.serve_seconds, 200.0, baseline_seconds=10.0,
+                                                      log=_log)
 
     kname = _step_kernel("double", main_res["homogeneous"], case.n[2])
     ttag = "fp64_auto" if main_res["homogeneous"] else "fp64_general"

@@ -140,6 +140,24 @@ int rt6d_cfl_dt(rt6d_ctx *ctx, double cfl, double *dt);
  * integrate refills them every step). */
 int rt6d_upload_E(rt6d_ctx *ctx, const double *E, int layout);
 
+/* E on the device from a separable product (no reference counterpart: the
+ * reference's config plug-ins fill E on the host in set_initial_conditions,
+ * e.g. for_config/gm1.f90).  factors = f_x(1..nx), f_y(1..ny), ..., f_m(1..nm)
+ * (n0+...+n5 values); E = peak*fx*fy*fz*fk*fl*fm*(1 + noise*(2u-1)) with the
+ * products taken left to right in binary64 and u = U[0,1) from splitmix64
+ * keyed by (seed, member, linear cell index), then rounded to the context's
+ * precision.  Replaces E like rt6d_upload_E. */
+int rt6d_init_separable(rt6d_ctx *ctx, const double *factors, double peak, double noise,
+                        unsigned long long seed, unsigned long long member);
+
+/* Superbee branch flips between a binary64 context and a binary32 context of
+ * the same problem (no reference counterpart: a precision study of the FP32
+ * form).  For the upper face of every cell and each advected axis, the
+ * limiter branch (phi = 0, 2r, 1, r or 2) that each context selects on its
+ * own current state in its own arithmetic; faces[d] counts the faces
+ * compared, flips[d] those whose branches differ. */
+int rt6d_branch_flips(rt6d_ctx *fp64_ctx, rt6d_ctx *fp32_ctx, long long faces[6], long long flips[6]);
+
 /* integrate (for_src/integrate.f90:4-271) called nsteps times with step dt. */
 int rt6d_step(rt6d_ctx *ctx, double dt, int nsteps);
 
@@ -182,8 +200,10 @@ int rt6d_synchronize(rt6d_ctx *ctx);
  * Ensembles (no reference counterpart: the reference runs one integration
  * per MPI job).  An ensemble advances many independent integrations -- each
  * a context set up as above (calc_parameter done, E uploaded), with its own
- * grid, background and dt -- in one persistent kernel launch per step over a
- * (member, block) work list, the steps replayed from a captured HIP graph.
+ * grid, background and dt -- with one launch per step and kernel variant
+ * over a (member, block) work list (one block per task, or persistent blocks
+ * taking tasks from work counters), two steps replayed from a captured HIP
+ * graph.
  * Members keep their own state; while in an ensemble a context must not be
  * stepped on its own.  rt6d_ensemble_remove (or destroy) hands the context
  * back with its state current, so rt6d_download/rt6d_diagnose work on it.
@@ -198,11 +218,21 @@ int rt6d_ensemble_add(rt6d_ensemble *ens, rt6d_ctx *member, double dt, int *slot
 int rt6d_ensemble_remove(rt6d_ensemble *ens, int slot);
 /* advance every member nsteps steps; *ms (may be NULL) gets the device time */
 int rt6d_ensemble_step(rt6d_ensemble *ens, int nsteps, double *ms);
-/* schedule: persistent blocks per compute unit (1..16) and work-list order
- * (0: member by member, 1: members interleaved block by block) */
+/* the same in two halves: enqueue the steps on the ensemble's stream and
+ * return, so that several ensembles run on the device at once; wait for
+ * them (*ms: device time since the first enqueue).  Any call on a member
+ * context first waits for its ensemble's enqueued steps. */
+int rt6d_ensemble_step_async(rt6d_ensemble *ens, int nsteps);
+int rt6d_ensemble_wait(rt6d_ensemble *ens, double *ms);
+/* schedule: blocks_per_cu 0 runs one hardware block per (member, block) task
+ * (default); 1..16 runs that many persistent blocks per compute unit, which
+ * take tasks from per-XCD work counters.  order 0: the work list member by
+ * member; 1: members interleaved block by block */
 int rt6d_ensemble_set_schedule(rt6d_ensemble *ens, int blocks_per_cu, int order);
-/* autotuner: times each schedule on `trial_steps` real steps of the current
- * members (results do not depend on the schedule) and keeps the fastest */
+/* autotuner: times each schedule (blocks_per_cu 0, 2, 4, 8 x both orders) on
+ * 2 + trial_steps real steps of the current members -- the members advance
+ * by 8*(2 + trial_steps) steps; results do not depend on the schedule -- and
+ * keeps the fastest */
 int rt6d_ensemble_tune(rt6d_ensemble *ens, int trial_steps, int *blocks_per_cu, int *order);
 /* number of members, total cells, steps taken by the ensemble */
 int rt6d_ensemble_info(rt6d_ensemble *ens, int *members, long long *cells, long long *steps);

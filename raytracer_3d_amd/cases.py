@@ -303,6 +303,57 @@ def member_case(seed: int, member: int, fp32: bool = False) -> Case:
     return c
 
 
+def separable_E(factors, peak: float, noise: float, seed: int, member: int) -> np.ndarray:
+    """E = peak*fx*fy*fz*fk*fl*fm*(1 + noise*(2u-1)), products left to right
+    in binary64, u = uniform(seed, member, linear index): the host statement of
+    rt6d_init_separable (Fortran order, x fastest)."""
+    A = peak * np.asarray(factors[0], dtype=np.float64)
+    for f in factors[1:]:
+        A = A[..., None] * np.asarray(f, dtype=np.float64)
+    E = A.ravel(order="F")
+    u = uniform(seed, member, np.arange(E.size, dtype=np.uint64))
+    return E * (1.0 + noise * (2.0 * u - 1.0))
+
+
+def member_spec(seed: int, member: int, n=None) -> Case:
+    """One ensemble or serving member without a host E: grid sizes from
+    {12,16,20,24} (x,y,z) and {8,12,16} (k,l,m) unless ``n`` is given,
+    N0 ~ U[2e-3, 8e-3] with N(z) = N0 exp(z/1300), U0 ~ U[0, 0.3] tanh shear,
+    a Gaussian packet in (x,z,k,m) with seeded centres and 1% noise, held as
+    separable factors (``meta["init"]``) for rt6d_init_separable."""
+    u = uniform(seed, member, np.arange(16, dtype=np.uint64), stream=7)
+    nxyz = [12, 16, 20, 24]
+    nklm = [8, 12, 16]
+    if n is None:
+        n = (nxyz[int(u[0] * 4)], nxyz[int(u[1] * 4)], nxyz[int(u[2] * 4)],
+             nklm[int(u[3] * 3)], nklm[int(u[4] * 3)], nklm[int(u[5] * 3)])
+    n = tuple(int(v) for v in n)
+    N0 = 2e-3 + 6e-3 * u[6]
+    U0 = 0.3 * u[7]
+    L = (100e3, 100e3, 4000.0)
+    x0 = L[0] * (0.25 + 0.5 * u[8])
+    z0 = -L[2] * (0.25 + 0.5 * u[9])
+    k0 = 2 * math.pi / (5e3 + 15e3 * u[10])
+    m0 = 2 * math.pi / (200.0 + 600.0 * u[11])
+    kmax, lmax, mmax = 2 * math.pi / 1000.0, 2 * math.pi / 1000.0, 2 * math.pi / 50.0
+    sp = [np.full(n[0], L[0] / n[0]), np.full(n[1], L[1] / n[1]), np.full(n[2], L[2] / n[2]),
+          _wavenumber_spacing(n[3], kmax), _wavenumber_spacing(n[4], lmax), _wavenumber_spacing(n[5], mmax)]
+    c = Case(f"member{member}", n, sp, None, None, None, None, f0=1e-4)
+    g = c.grids()
+    c.Nsqr = _halo3(n, lambda X, Y, Z: (N0 * np.exp(Z / 1300.0)) ** 2, g)
+    c.U = _halo3(n, lambda X, Y, Z: U0 * np.tanh((Z + 500.0) / 300.0), g)
+    c.V = np.zeros_like(c.Nsqr)
+    xs, ys, zs, ks, ls, ms = _interior_axes(n, g)
+    sx, sz = 10e3, 500.0
+    factors = [np.exp(-((xs - x0) ** 2) / (2 * sx * sx)), np.ones(n[1]),
+               np.exp(-((zs - z0) ** 2) / (2 * sz * sz)),
+               np.exp(-((ks - k0) ** 2) / (2 * (0.3 * k0) ** 2)), np.ones(n[4]),
+               np.exp(-((ms - m0) ** 2) / (2 * (0.3 * m0) ** 2))]
+    c.meta = dict(kind="member_spec", seed=seed, member=member, N0=N0, U0=U0, x0=x0, z0=z0, k0=k0, m0=m0,
+                  init=dict(factors=factors, peak=1e-3, noise=0.01, seed=seed, member=member))
+    return c
+
+
 # ---------------------------------------------------------------- case file
 
 

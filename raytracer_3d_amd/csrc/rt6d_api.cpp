@@ -120,11 +120,15 @@ struct rt6d_ctx {
 
 namespace {
 
+int ens_settle(rt6d_ensemble *e);
+
 int check_ctx(rt6d_ctx *c)
 {
     if (!c) return fail(RT6D_EINVAL, "null context");
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return fail(RT6D_EHIP, "hipSetDevice: %s", hipGetErrorString(e));
+    // an ensemble member's state is current once the ensemble's enqueued steps are done
+    if (c->owner) return ens_settle(c->owner);
     return 0;
 }
 
@@ -212,6 +216,7 @@ int not_member(const rt6d_ctx *c, const char *fn)
 }
 
 void ens_detach(rt6d_ensemble *e, rt6d_ctx *c);
+int ens_settle(rt6d_ensemble *e);
 
 int require_ready(rt6d_ctx *c)
 {
@@ -685,6 +690,68 @@ int rt6d_upload_E(rt6d_ctx *c, const double *E, int layout)
     return 0;
 }
 
+int rt6d_init_separable(rt6d_ctx *c, const double *factors, double peak, double noise, unsigned long long seed,
+                        unsigned long long member)
+{
+    int r;
+    if ((r = check_ctx(c))) return r;
+    if ((r = not_member(c, "rt6d_init_separable"))) return r;
+    if (!factors || !std::isfinite(peak) || !std::isfinite(noise))
+        return fail(RT6D_EINVAL, "rt6d_init_separable: bad arguments");
+    size_t nf = 0;
+    for (int d = 0; d < kAxes; ++d) nf += (size_t)c->n[d];
+    double *d_f = nullptr;
+    if ((r = c->dalloc((void **)&d_f, nf * sizeof(double)))) return r;
+    // the host generator's key for stream 0: seed*FNV + member*golden (mod 2^64)
+    const uint64_t key = (uint64_t)seed * 0x100000001B3ull + (uint64_t)member * 0x2545F4914F6CDD1Dull;
+    hipError_t e = hipMemcpyAsync(d_f, factors, nf * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = c->prec == RT6D_FP64
+                ? launch_init_separable<double>(c->g, d_f, peak, noise, key, (double *)c->d_E[c->cur], c->stream)
+                : launch_init_separable<float>(c->g, d_f, peak, noise, key, (float *)c->d_E[c->cur], c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    c->dfree((void *&)d_f);
+    c->dev_bytes -= nf * sizeof(double);
+    HIPCHK(e);
+    c->E_set = true;
+    c->have_prev = false;
+    c->steps = 0;
+    c->fl_src = c->de_src = 0;
+    return 0;
+}
+
+int rt6d_branch_flips(rt6d_ctx *a, rt6d_ctx *b, long long faces[6], long long flips[6])
+{
+    int r;
+    if (!a || !b || !faces || !flips) return fail(RT6D_EINVAL, "rt6d_branch_flips: null argument");
+    if (a->prec != RT6D_FP64 || b->prec != RT6D_FP32)
+        return fail(RT6D_EINVAL, "rt6d_branch_flips: needs a binary64 and a binary32 context");
+    for (int d = 0; d < kAxes; ++d)
+        if (a->n[d] != b->n[d]) return fail(RT6D_EINVAL, "rt6d_branch_flips: grids differ");
+    if (a->device != b->device) return fail(RT6D_EINVAL, "rt6d_branch_flips: contexts on different devices");
+    if ((r = check_ctx(a))) return r;
+    if ((r = require_ready(a)) || (r = require_ready(b))) return r;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    unsigned long long *d_cnt = nullptr;
+    if ((r = a->dalloc((void **)&d_cnt, 12 * sizeof(unsigned long long)))) return r;
+    const double dt = std::isfinite(a->coef_dt) ? a->coef_dt : 1.0;
+    const StepPtrs<double> pa = step_ptrs<double>(a, a->d_E[a->cur], nullptr, dt);
+    const StepPtrs<float> pb = step_ptrs<float>(b, b->d_E[b->cur], nullptr, dt);
+    unsigned long long h[12] = {0};
+    hipError_t e = hipMemsetAsync(d_cnt, 0, sizeof h, a->stream);
+    if (e == hipSuccess) e = launch_branch_flips(a->g, pa, a->homog, pb, b->homog, d_cnt, a->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(h, d_cnt, sizeof h, hipMemcpyDeviceToHost, a->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(a->stream);
+    a->dfree((void *&)d_cnt);
+    a->dev_bytes -= sizeof h;
+    HIPCHK(e);
+    for (int d = 0; d < kAxes; ++d) {
+        faces[d] = (long long)h[d];
+        flips[d] = (long long)h[6 + d];
+    }
+    return 0;
+}
+
 static int do_steps(rt6d_ctx *c, double dt, int nsteps)
 {
     int r;
@@ -1017,13 +1084,16 @@ struct rt6d_ensemble {
     std::vector<rt6d_ctx *> slot;     // nullptr = free
     std::vector<double> dt;
     std::vector<int> cur0;            // member's current buffer at the last rebuild
-    int blocks_per_cu = 4, order = 0, ncu = 256;
+    int blocks_per_cu = 0, order = 0, ncu = 256;
     void *d_mem = nullptr;
     size_t mem_cap = 0;
     EnsTask *d_tasks = nullptr;
     size_t task_cap = 0;
     uint32_t ntasks = 0;
+    uint32_t off[kEnsVariants] = {0}, cnt[kEnsVariants] = {0};
+    uint32_t *d_ctr = nullptr;
     bool dirty = true;
+    bool pending = false;             // steps enqueued, not yet waited for
     long long k = 0;                  // steps since the last rebuild
     long long steps = 0;
     hipGraphExec_t graph[2] = {nullptr, nullptr};   // two steps starting at parity 0 / 1
@@ -1031,14 +1101,22 @@ struct rt6d_ensemble {
 
 namespace {
 
+void ens_drop_graphs(rt6d_ensemble *e)
+{
+    for (int q = 0; q < 2; ++q)
+        if (e->graph[q]) { hipGraphExecDestroy(e->graph[q]); e->graph[q] = nullptr; }
+}
+
 template <typename T>
 int ens_rebuild_t(rt6d_ensemble *e)
 {
     std::vector<EnsMember<T>> mem;
     std::vector<uint32_t> nbl;
+    std::vector<int> var;
     for (size_t i = 0; i < e->slot.size(); ++i) {
         EnsMember<T> M{};
         rt6d_ctx *c = e->slot[i];
+        int v = 0;
         if (c) {
             int zc = 1;
             if (!col_config<T>(c->g, M.cc, zc)) return fail(RT6D_EINVAL, "ensemble member too large");
@@ -1047,35 +1125,46 @@ int ens_rebuild_t(rt6d_ensemble *e)
             M.p[0] = step_ptrs<T>(c, c->d_E[a], c->d_E[a ^ 1], e->dt[i]);
             M.p[1] = step_ptrs<T>(c, c->d_E[a ^ 1], c->d_E[a], e->dt[i]);
             const bool full = c->homog ? is_full<true>(c->g) : is_full<false>(c->g);
-            M.variant = (c->homog ? 1u : 0u) | (zc == 2 ? 2u : 0u) | (full ? 4u : 0u);
+            v = (c->homog ? 1 : 0) | (zc == 2 ? 2 : 0) | (full ? 4 : 0);
+            M.variant = (uint32_t)v;
             e->cur0[i] = a;
             nbl.push_back(col_blocks<T>(c->g, M.cc));
         } else {
             nbl.push_back(0);
         }
+        var.push_back(v);
         mem.push_back(M);
     }
+    // the work list, grouped by variant; within a variant member by member or interleaved
     std::vector<EnsTask> tasks;
-    if (e->order == 0) {
-        for (size_t i = 0; i < nbl.size(); ++i)
-            for (uint32_t b = 0; b < nbl[i]; ++b) tasks.push_back(EnsTask{(uint32_t)i, b});
-    } else {
-        uint32_t mx = 0;
-        for (uint32_t v : nbl) mx = v > mx ? v : mx;
-        for (uint32_t b = 0; b < mx; ++b)
+    for (int v = 0; v < kEnsVariants; ++v) {
+        e->off[v] = (uint32_t)tasks.size();
+        if (e->order == 0) {
             for (size_t i = 0; i < nbl.size(); ++i)
-                if (b < nbl[i]) tasks.push_back(EnsTask{(uint32_t)i, b});
+                if (var[i] == v)
+                    for (uint32_t b = 0; b < nbl[i]; ++b) tasks.push_back(EnsTask{(uint32_t)i, b});
+        } else {
+            uint32_t mx = 0;
+            for (size_t i = 0; i < nbl.size(); ++i)
+                if (var[i] == v && nbl[i] > mx) mx = nbl[i];
+            for (uint32_t b = 0; b < mx; ++b)
+                for (size_t i = 0; i < nbl.size(); ++i)
+                    if (var[i] == v && b < nbl[i]) tasks.push_back(EnsTask{(uint32_t)i, b});
+        }
+        e->cnt[v] = (uint32_t)tasks.size() - e->off[v];
     }
     const size_t mb = mem.size() * sizeof(EnsMember<T>), tb = tasks.size() * sizeof(EnsTask);
     if (mb > e->mem_cap) {
         if (e->d_mem) hipFree(e->d_mem);
         e->d_mem = nullptr;
+        e->mem_cap = 0;
         HIPCHK(hipMalloc(&e->d_mem, mb));
         e->mem_cap = mb;
     }
     if (tb > e->task_cap) {
         if (e->d_tasks) hipFree(e->d_tasks);
         e->d_tasks = nullptr;
+        e->task_cap = 0;
         HIPCHK(hipMalloc((void **)&e->d_tasks, tb));
         e->task_cap = tb;
     }
@@ -1083,8 +1172,7 @@ int ens_rebuild_t(rt6d_ensemble *e)
     if (tb) HIPCHK(hipMemcpyAsync(e->d_tasks, tasks.data(), tb, hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     e->ntasks = (uint32_t)tasks.size();
-    for (int q = 0; q < 2; ++q)
-        if (e->graph[q]) { hipGraphExecDestroy(e->graph[q]); e->graph[q] = nullptr; }
+    ens_drop_graphs(e);
     e->k = 0;
     e->dirty = false;
     return 0;
@@ -1102,14 +1190,14 @@ void ens_sync_members(rt6d_ensemble *e)
 
 int ens_launch(rt6d_ensemble *e, uint32_t parity)
 {
-    const uint32_t nchunks = (uint32_t)(e->blocks_per_cu * e->ncu);
+    const uint32_t pb = e->blocks_per_cu > 0 ? (uint32_t)(e->blocks_per_cu * e->ncu) : 0u;
     hipError_t r;
     if (e->prec == RT6D_FP64)
-        r = launch_ensemble<double>((const EnsMember<double> *)e->d_mem, e->d_tasks, e->ntasks, parity, nchunks,
-                                    e->stream);
+        r = launch_ensemble<double>((const EnsMember<double> *)e->d_mem, e->d_tasks, e->off, e->cnt, parity, pb,
+                                    e->d_ctr, e->stream);
     else
-        r = launch_ensemble<float>((const EnsMember<float> *)e->d_mem, e->d_tasks, e->ntasks, parity, nchunks,
-                                   e->stream);
+        r = launch_ensemble<float>((const EnsMember<float> *)e->d_mem, e->d_tasks, e->off, e->cnt, parity, pb,
+                                   e->d_ctr, e->stream);
     HIPCHK(r);
     return 0;
 }
@@ -1133,18 +1221,31 @@ int ens_graph(rt6d_ensemble *e, int start)
     return 0;
 }
 
-int ens_steps(rt6d_ensemble *e, int nsteps)
+// Wait for enqueued steps (if any) and settle the members' state.
+int ens_settle(rt6d_ensemble *e)
+{
+    if (e->pending) {
+        HIPCHK(hipStreamSynchronize(e->stream));
+        e->pending = false;
+    }
+    ens_sync_members(e);
+    return 0;
+}
+
+int ens_enqueue(rt6d_ensemble *e, int nsteps)
 {
     int r;
     if (e->dirty) {
-        ens_sync_members(e);
+        if ((r = ens_settle(e))) return r;
         r = e->prec == RT6D_FP64 ? ens_rebuild_t<double>(e) : ens_rebuild_t<float>(e);
         if (r) return r;
     }
-    if (e->ntasks == 0) return 0;
-    // a member's dx/dt coefficients follow the last dt used on it (diagnose may use another)
-    for (size_t i = 0; i < e->slot.size(); ++i)
-        if (e->slot[i] && (r = ensure_coef(e->slot[i], e->dt[i]))) return r;
+    if (e->ntasks == 0 || nsteps == 0) return 0;
+    if (!e->pending) {
+        // a member's dx/dt coefficients follow the last dt used on it (diagnose may use another)
+        for (size_t i = 0; i < e->slot.size(); ++i)
+            if (e->slot[i] && (r = ensure_coef(e->slot[i], e->dt[i]))) return r;
+    }
     int left = nsteps;
     while (left >= 2) {
         const int start = (int)(e->k & 1);
@@ -1157,6 +1258,7 @@ int ens_steps(rt6d_ensemble *e, int nsteps)
         if ((r = ens_launch(e, (uint32_t)(e->k & 1)))) return r;
         e->k += 1;
     }
+    e->pending = true;
     e->steps += nsteps;
     for (size_t i = 0; i < e->slot.size(); ++i) {
         rt6d_ctx *c = e->slot[i];
@@ -1166,7 +1268,6 @@ int ens_steps(rt6d_ensemble *e, int nsteps)
         c->prev_dt = e->dt[i];
         c->fl_src = c->de_src = 0;
     }
-    ens_sync_members(e);
     return 0;
 }
 
@@ -1194,9 +1295,10 @@ int rt6d_ensemble_create(int precision, rt6d_ensemble **out)
     if (hipGetDeviceProperties(&prop, e->device) == hipSuccess && prop.multiProcessorCount > 0)
         e->ncu = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess) {
+        hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
+        hipMalloc((void **)&e->d_ctr, kEnsCounters * sizeof(uint32_t)) != hipSuccess) {
         rt6d_ensemble_destroy(e);
-        return fail(RT6D_EHIP, "rt6d_ensemble_create: stream/event creation failed");
+        return fail(RT6D_EHIP, "rt6d_ensemble_create: stream/event/counter allocation failed");
     }
     *out = e;
     return 0;
@@ -1207,13 +1309,14 @@ int rt6d_ensemble_destroy(rt6d_ensemble *e)
     if (!e) return 0;
     hipSetDevice(e->device);
     if (e->stream) hipStreamSynchronize(e->stream);
+    e->pending = false;
     ens_sync_members(e);
     for (rt6d_ctx *c : e->slot)
         if (c) c->owner = nullptr;
-    for (int q = 0; q < 2; ++q)
-        if (e->graph[q]) hipGraphExecDestroy(e->graph[q]);
+    ens_drop_graphs(e);
     if (e->d_mem) hipFree(e->d_mem);
     if (e->d_tasks) hipFree(e->d_tasks);
+    if (e->d_ctr) hipFree(e->d_ctr);
     if (e->ev0) hipEventDestroy(e->ev0);
     if (e->ev1) hipEventDestroy(e->ev1);
     if (e->stream) hipStreamDestroy(e->stream);
@@ -1236,11 +1339,10 @@ int rt6d_ensemble_add(rt6d_ensemble *e, rt6d_ctx *c, double dt, int *slot)
     int zc;
     const bool fits = e->prec == RT6D_FP64 ? col_config<double>(c->g, cc, zc) : col_config<float>(c->g, cc, zc);
     if (!fits) return fail(RT6D_EINVAL, "rt6d_ensemble_add: member too large for the ensemble kernel");
+    // finish enqueued ensemble steps before the member set changes
+    if ((r = ens_settle(e))) return r;
     if ((r = ensure_coef(c, dt))) return r;
     HIPCHK(hipStreamSynchronize(c->stream));
-    // finish pending ensemble steps before the member set changes
-    HIPCHK(hipStreamSynchronize(e->stream));
-    ens_sync_members(e);
     int s = -1;
     for (size_t i = 0; i < e->slot.size(); ++i)
         if (!e->slot[i]) { s = (int)i; break; }
@@ -1263,42 +1365,57 @@ int rt6d_ensemble_remove(rt6d_ensemble *e, int s)
 {
     if (!e || s < 0 || s >= (int)e->slot.size() || !e->slot[s])
         return fail(RT6D_EINVAL, "rt6d_ensemble_remove: bad slot %d", s);
-    HIPCHK(hipStreamSynchronize(e->stream));
-    ens_sync_members(e);
+    int r;
+    if ((r = ens_settle(e))) return r;
     e->slot[s]->owner = nullptr;
     e->slot[s] = nullptr;
     e->dirty = true;
     return 0;
 }
 
-int rt6d_ensemble_step(rt6d_ensemble *e, int nsteps, double *ms)
+int rt6d_ensemble_step_async(rt6d_ensemble *e, int nsteps)
 {
-    if (!e || nsteps < 0) return fail(RT6D_EINVAL, "rt6d_ensemble_step: bad arguments");
+    if (!e || nsteps < 0) return fail(RT6D_EINVAL, "rt6d_ensemble_step_async: bad arguments");
     HIPCHK(hipSetDevice(e->device));
-    HIPCHK(hipEventRecord(e->ev0, e->stream));
-    int r = ens_steps(e, nsteps);
+    if (!e->pending) HIPCHK(hipEventRecord(e->ev0, e->stream));
+    int r = ens_enqueue(e, nsteps);
     if (r) return r;
-    HIPCHK(hipEventRecord(e->ev1, e->stream));
-    HIPCHK(hipEventSynchronize(e->ev1));
-    if (ms) {
+    if (e->pending) HIPCHK(hipEventRecord(e->ev1, e->stream));
+    return 0;
+}
+
+int rt6d_ensemble_wait(rt6d_ensemble *e, double *ms)
+{
+    if (!e) return fail(RT6D_EINVAL, "rt6d_ensemble_wait: null ensemble");
+    HIPCHK(hipSetDevice(e->device));
+    double t = 0.0;
+    if (e->pending) {
+        HIPCHK(hipEventSynchronize(e->ev1));
         float f = 0.f;
         HIPCHK(hipEventElapsedTime(&f, e->ev0, e->ev1));
-        *ms = (double)f;
+        t = (double)f;
     }
+    int r = ens_settle(e);
+    if (r) return r;
+    if (ms) *ms = t;
     return 0;
+}
+
+int rt6d_ensemble_step(rt6d_ensemble *e, int nsteps, double *ms)
+{
+    int r;
+    if ((r = rt6d_ensemble_step_async(e, nsteps))) return r;
+    return rt6d_ensemble_wait(e, ms);
 }
 
 int rt6d_ensemble_set_schedule(rt6d_ensemble *e, int blocks_per_cu, int order)
 {
-    if (!e || blocks_per_cu < 1 || blocks_per_cu > 16 || order < 0 || order > 1)
+    if (!e || blocks_per_cu < 0 || blocks_per_cu > 16 || order < 0 || order > 1)
         return fail(RT6D_EINVAL, "rt6d_ensemble_set_schedule: bad arguments");
-    HIPCHK(hipStreamSynchronize(e->stream));
-    ens_sync_members(e);
+    int r;
+    if ((r = ens_settle(e))) return r;
     if (order != e->order) e->dirty = true;
-    if (blocks_per_cu != e->blocks_per_cu) {
-        for (int q = 0; q < 2; ++q)
-            if (e->graph[q]) { hipGraphExecDestroy(e->graph[q]); e->graph[q] = nullptr; }
-    }
+    if (blocks_per_cu != e->blocks_per_cu) ens_drop_graphs(e);
     e->blocks_per_cu = blocks_per_cu;
     e->order = order;
     return 0;
@@ -1307,7 +1424,7 @@ int rt6d_ensemble_set_schedule(rt6d_ensemble *e, int blocks_per_cu, int order)
 int rt6d_ensemble_tune(rt6d_ensemble *e, int trial_steps, int *bpc_out, int *order_out)
 {
     if (!e || trial_steps < 2) return fail(RT6D_EINVAL, "rt6d_ensemble_tune: need >= 2 trial steps");
-    const int bpcs[] = {2, 4, 8};
+    const int bpcs[] = {0, 2, 4, 8};
     double best = 1e300;
     int bb = e->blocks_per_cu, bo = e->order;
     for (int order = 0; order <= 1; ++order)

@@ -8,12 +8,9 @@
 //    m is carried as the left face of step m+1;
 //  * the ZC cells share their ZC-1 inner z faces; the four outer z
 //    neighbours are read one plane ahead;
-//  * x neighbours come from the neighbouring lanes (lane = x + nx*y within
-//    the wavefront) by DPP lane shifts; only the two lanes at either end of
-//    a wavefront load theirs.  y neighbours are plain loads (L1/L2 hits:
-//    the neighbouring lanes read the same lines).  Both faces are computed
-//    per cell.  The vector-memory pipeline, not HBM or VALU, bounds this
-//    kernel, so every load taken off it counts;
+//  * x and y neighbours are plain loads (L1/L2 hits: the neighbouring lanes
+//    read the same lines), or, with LDSY, the y neighbours come from the
+//    block's planes staged in LDS; both faces are computed per cell;
 //  * the (x,y) index is padded to a multiple of 64 so that a wavefront never
 //    spans two (z,k,l) planes.  Every access is then a raw buffer access with
 //    a wave-uniform byte offset (scalar register) and a per-thread 32-bit
@@ -75,25 +72,6 @@ __device__ __forceinline__ T sld(const T *p, uint32_t i)
     return ((const __attribute__((address_space(4))) T *)p)[i];
 }
 
-// Whole-wavefront lane shifts (DPP wave_shr:1 / wave_shl:1): lane i receives
-// v of lane i-1 (shr) or i+1 (shl); lanes 0 and 63 keep their own v.
-__device__ __forceinline__ int dpp_shr1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
-__device__ __forceinline__ int dpp_shl1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false); }
-template <bool SHR>
-__device__ __forceinline__ double lane_shift(double v)
-{
-    const uint64_t u = __builtin_bit_cast(uint64_t, v);
-    const int lo = (int)(uint32_t)u, hi = (int)(uint32_t)(u >> 32);
-    const int rl = SHR ? dpp_shr1(lo) : dpp_shl1(lo), rh = SHR ? dpp_shr1(hi) : dpp_shl1(hi);
-    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)rh << 32) | (uint32_t)rl);
-}
-template <bool SHR>
-__device__ __forceinline__ float lane_shift(float v)
-{
-    const int i = __builtin_bit_cast(int, v);
-    return __builtin_bit_cast(float, SHR ? dpp_shr1(i) : dpp_shl1(i));
-}
-
 // Axis mask of the common case in which every axis that the path advects is
 // active (x, y, z, m on the homogeneous path; all six on the general path).
 // With FULL the kernel is compiled for that mask, without branches on it.
@@ -135,7 +113,6 @@ __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, con
     const int x = x0 + 1, y = y0 + 1;
     const int z0 = zb * ZC + 1, k = bk + 1, l = bl + 1;
     constexpr uint32_t S = sizeof(T);
-    const uint32_t lane = threadIdx.x & 63u;   // = xy mod 64: planes start on wavefront boundaries
 
     const size_t s5 = (size_t)g.inner[5];
     const uint32_t nzkl = (uint32_t)nz * nk * nl;
@@ -294,20 +271,8 @@ __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, con
             const T e0 = w0[j];
             T dE = T(0);
             if (ax) {
-                // E(x-2), E(x-1), E(x+1), E(x+2) of the same plane: the lanes' own e0
-                T a1 = lane_shift<true>(e0), a3 = lane_shift<false>(e0);
-                T a0 = lane_shift<true>(a1), a4 = lane_shift<false>(a3);
-                if (lane < 2 || lane >= 62) {   // neighbours held by the adjacent wavefronts
-                    if (lane == 0) a1 = bld<T>(rE, ox[1], pb);
-                    if (lane < 2) a0 = bld<T>(rE, ox[0], pb);
-                    if (lane == 63) a3 = bld<T>(rE, ox[2], pb);
-                    if (lane >= 62) a4 = bld<T>(rE, ox[3], pb);
-                }
-                // zero-gradient ghosts at x = 1 and x = nx
-                a1 = x == 1 ? e0 : a1;
-                a0 = x <= 2 ? a1 : a0;
-                a3 = x == nx ? e0 : a3;
-                a4 = x >= nx - 1 ? a3 : a4;
+                const T a0 = bld<T>(rE, ox[0], pb), a1 = bld<T>(rE, ox[1], pb);
+                const T a3 = bld<T>(rE, ox[2], pb), a4 = bld<T>(rE, ox[3], pb);
                 T vL, vR;
                 if (HOMOG) {
                     const T v0 = sld(p.hom.vx3, q2), vi = sld(p.hom.vx3, p.hom.n4 + q2), vb = sld(p.hom.vx3, 2 * p.hom.n4 + q2);

@@ -1,12 +1,21 @@
-// rt6d_ensemble.h -- one launch steps many independent integrations
-// (included by rt6d_kernels.h after rt6d_column.h, inside namespace rt6d).
+// rt6d_ensemble.h -- one launch per kernel variant steps many independent
+// integrations (included by rt6d_kernels.h after rt6d_column.h, inside
+// namespace rt6d).
 //
 // Each member is described by its geometry, its two ping-pong orders of step
 // pointers and its column-kernel configuration.  The work list holds one
-// (member, logical block) task per 256-thread block of every member.  A
-// persistent grid of `nchunks` blocks splits the list into contiguous chunks
-// (chunk ids are XCD-aware, so one XCD works on neighbouring planes of the
-// same members) and runs the column-kernel body of each task in turn.
+// (member, logical block) task per 256-thread block of every member, grouped
+// by kernel variant (homogeneous or general coefficients, one or two z cells
+// per thread, all advected axes active or not), so that every launch is
+// compiled for exactly one variant and keeps that variant's register budget.
+// Two ways to run a variant's tasks:
+//  * dispatch: one hardware block per task, in list order; block ids are
+//    XCD-aware, so each XCD works on a contiguous stretch of the list (the
+//    neighbouring blocks of one member share that XCD's L2);
+//  * persistent: a grid of blocks_per_cu blocks per compute unit; the blocks
+//    of each XCD take the tasks of that XCD's eighth of the list from a work
+//    counter until it is exhausted (every block reaches the end).  The
+//    counters are cleared by a small kernel at the start of every step.
 #pragma once
 
 template <typename T>
@@ -14,8 +23,7 @@ struct EnsMember {
     Geo g;
     StepPtrs<T> p[2];   // p[0]: E[a] -> E[b]; p[1]: E[b] -> E[a]
     ColCfg cc;
-    uint32_t variant;   // bit 0: horizontally homogeneous tables; bit 1: two z cells per thread;
-                        // bit 2: every advected axis active
+    uint32_t variant;   // bit 0: homogeneous tables; bit 1: two z cells per thread; bit 2: all axes active
     uint32_t pad;
 };
 
@@ -23,28 +31,40 @@ struct EnsTask {
     uint32_t member, block;
 };
 
-template <typename T>
+constexpr int kEnsVariants = 8;
+constexpr int kEnsCounters = kEnsVariants * 8;   // one work counter per variant and XCD
+
+static __global__ void k_ens_reset(uint32_t *ctr)
+{
+    if (threadIdx.x < kEnsCounters) ctr[threadIdx.x] = 0;
+}
+
+template <typename T, bool HOMOG, int ZC, bool FULL, bool PERSIST>
 __global__ void __launch_bounds__(256) k_ens_step(const EnsMember<T> *__restrict__ mem,
                                                   const EnsTask *__restrict__ tasks, uint32_t ntasks,
-                                                  uint32_t parity, uint32_t nchunks)
+                                                  uint32_t parity, uint32_t *ctr)
 {
-    const uint32_t c = xcd_logical_block(blockIdx.x, nchunks);
-    if (c >= nchunks) return;
-    const uint32_t t0 = (uint32_t)(((uint64_t)ntasks * c) / nchunks);
-    const uint32_t t1 = (uint32_t)(((uint64_t)ntasks * (c + 1)) / nchunks);
-    for (uint32_t t = t0; t < t1; ++t) {
+    if constexpr (!PERSIST) {
+        const uint32_t t = xcd_logical_block(blockIdx.x, ntasks);
+        if (t >= ntasks) return;   // uniform over the block
         const EnsTask tk = tasks[t];
         const EnsMember<T> &M = mem[tk.member];
-        const StepPtrs<T> &P = M.p[parity & 1u];
-        switch (M.variant) {
-        case 0: col_body<T, false, false, 1, false>(M.g, P, M.cc, tk.block); break;
-        case 1: col_body<T, false, true, 1, false>(M.g, P, M.cc, tk.block); break;
-        case 2: col_body<T, false, false, 2, false>(M.g, P, M.cc, tk.block); break;
-        case 3: col_body<T, false, true, 2, false>(M.g, P, M.cc, tk.block); break;
-        case 4: col_body<T, false, false, 1, true>(M.g, P, M.cc, tk.block); break;
-        case 5: col_body<T, false, true, 1, true>(M.g, P, M.cc, tk.block); break;
-        case 6: col_body<T, false, false, 2, true>(M.g, P, M.cc, tk.block); break;
-        default: col_body<T, false, true, 2, true>(M.g, P, M.cc, tk.block); break;
+        col_body<T, false, HOMOG, ZC, FULL>(M.g, M.p[parity & 1u], M.cc, tk.block);
+    } else {
+        __shared__ uint32_t s_task;
+        const uint32_t xcd = blockIdx.x & 7u;
+        const uint32_t per = (ntasks + 7u) / 8u;
+        const uint32_t t0 = xcd * per;
+        const uint32_t t1 = t0 + per < ntasks ? t0 + per : ntasks;
+        for (;;) {
+            if (threadIdx.x == 0) s_task = t0 + atomicAdd(&ctr[xcd], 1u);
+            __syncthreads();
+            const uint32_t t = s_task;
+            __syncthreads();   // every thread has read s_task before thread 0 writes it again
+            if (t >= t1) break;
+            const EnsTask tk = tasks[t];
+            const EnsMember<T> &M = mem[tk.member];
+            col_body<T, false, HOMOG, ZC, FULL>(M.g, M.p[parity & 1u], M.cc, tk.block);
         }
     }
 }

@@ -348,35 +348,56 @@ __device__ __forceinline__ void atomic_max_pos(unsigned long long *bits, double 
     atomicMax(bits, (unsigned long long)__double_as_longlong(v));
 }
 
+// Largest |velocity| per axis at the reference's T points and the largest
+// CFL sum: grid-stride loops, a wavefront then a block reduction, one atomic
+// per block and quantity (max is exact, so the order does not matter).
+__device__ __forceinline__ double block_max(double v, double *red)
+{
+    v = wave_max(v);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    double r = red[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) r = fmax(r, red[i]);
+    return r;
+}
+
 __global__ void __launch_bounds__(256) k_vmax(const Geo g, const double *__restrict__ v0, const double *__restrict__ v1,
                                               const double *__restrict__ v2, const double *__restrict__ v3,
                                               const double *__restrict__ v4, const double *__restrict__ v5,
                                               unsigned long long *bits)
 {
-    const uint32_t lin = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double red[4];
     double m[kAxes] = {0, 0, 0, 0, 0, 0};
-    if (lin < g.ncells) {
+    const double *vv[kAxes] = {v0, v1, v2, v3, v4, v5};
+    for (uint32_t lin = blockIdx.x * blockDim.x + threadIdx.x; lin < g.ncells; lin += gridDim.x * blockDim.x) {
         int c[kAxes];
         uint32_t q[kAxes + 1];
         decode(g, lin, c, q);
-        const double *vv[kAxes] = {v0, v1, v2, v3, v4, v5};
 #pragma unroll
         for (int d = 0; d < kAxes; ++d) {
             // T point c of the reference = right face of cell c
             const uint32_t fr = lin + (uint32_t)g.inner[d] * (q[d + 1] + 1);
-            m[d] = fabs(vv[d][fr]);
+            m[d] = fmax(m[d], fabs(vv[d][fr]));
         }
     }
 #pragma unroll
     for (int d = 0; d < kAxes; ++d) {
-        const double w = wave_max(m[d]);
-        if ((threadIdx.x & 63) == 0) atomic_max_pos(&bits[d], w);
+        const double w = block_max(m[d], red);
+        if (threadIdx.x == 0) atomic_max_pos(&bits[d], w);
     }
+}
+
+static uint32_t reduce_blocks(uint64_t n)
+{
+    const uint32_t nb = nblocks(n, 256);
+    return nb < 2048u ? (nb ? nb : 1u) : 2048u;
 }
 
 hipError_t launch_vmax(const Geo &g, const double *const vel[kAxes], unsigned long long *vmax_bits, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_vmax, dim3(nblocks(g.ncells, 256)), dim3(256), 0, s, g, vel[0], vel[1], vel[2], vel[3],
+    hipLaunchKernelGGL(k_vmax, dim3(reduce_blocks(g.ncells)), dim3(256), 0, s, g, vel[0], vel[1], vel[2], vel[3],
                        vel[4], vel[5], vmax_bits);
     return hipGetLastError();
 }
@@ -386,13 +407,14 @@ __global__ void __launch_bounds__(256) k_cfl_max(const Geo g, const double *__re
                                                  const double *__restrict__ v4, const double *__restrict__ v5,
                                                  const AxisCoef ax, unsigned long long *bits)
 {
-    const uint32_t lin = blockIdx.x * blockDim.x + threadIdx.x;
-    double s = 0.0;
-    if (lin < g.ncells) {
+    __shared__ double red[4];
+    const double *vv[kAxes] = {v0, v1, v2, v3, v4, v5};
+    double mx = 0.0;
+    for (uint32_t lin = blockIdx.x * blockDim.x + threadIdx.x; lin < g.ncells; lin += gridDim.x * blockDim.x) {
         int c[kAxes];
         uint32_t q[kAxes + 1];
         decode(g, lin, c, q);
-        const double *vv[kAxes] = {v0, v1, v2, v3, v4, v5};
+        double s = 0.0;
 #pragma unroll
         for (int d = 0; d < kAxes; ++d) {
             if (!(g.act & (1 << d))) continue;
@@ -400,18 +422,67 @@ __global__ void __launch_bounds__(256) k_cfl_max(const Geo g, const double *__re
             const double a = fabs(vv[d][fl]), b = fabs(vv[d][fl + (uint32_t)g.inner[d]]);
             s += (a > b ? a : b) / ax.dx[d][c[d]];
         }
+        mx = fmax(mx, s);
     }
-    const double w = wave_max(s);
-    if ((threadIdx.x & 63) == 0) atomic_max_pos(bits, w);
+    const double w = block_max(mx, red);
+    if (threadIdx.x == 0) atomic_max_pos(bits, w);
 }
 
 hipError_t launch_cfl_max(const Geo &g, const double *const vel[kAxes], const AxisCoef &ax, unsigned long long *bits,
                           hipStream_t s)
 {
-    hipLaunchKernelGGL(k_cfl_max, dim3(nblocks(g.ncells, 256)), dim3(256), 0, s, g, vel[0], vel[1], vel[2], vel[3],
+    hipLaunchKernelGGL(k_cfl_max, dim3(reduce_blocks(g.ncells)), dim3(256), 0, s, g, vel[0], vel[1], vel[2], vel[3],
                        vel[4], vel[5], ax, bits);
     return hipGetLastError();
 }
+
+// Separable initial state E = peak*fx*fy*fz*fk*fl*fm*(1 + noise*(2u-1)),
+// products left to right, u = U[0,1) from splitmix64 keyed by (seed, member,
+// linear cell index): the host generator's uniform(seed, member, idx, 0).
+// Built without contraction, so every product rounds as on the host.
+__device__ __forceinline__ uint64_t splitmix64_dev(uint64_t x)
+{
+    uint64_t z = x + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_init_separable(const Geo g, const double *__restrict__ f, double peak,
+                                                        double noise, uint64_t key, T *__restrict__ E)
+{
+#pragma clang fp contract(off)
+    for (uint32_t lin = blockIdx.x * blockDim.x + threadIdx.x; lin < g.ncells; lin += gridDim.x * blockDim.x) {
+        int c[kAxes];
+        uint32_t q[kAxes + 1];
+        decode(g, lin, c, q);
+        double v = peak;
+        uint32_t off = 0;
+#pragma unroll
+        for (int d = 0; d < kAxes; ++d) {
+            v = v * f[off + (uint32_t)(c[d] - 1)];
+            off += (uint32_t)g.n[d];
+        }
+        const uint64_t h = splitmix64_dev((uint64_t)lin * 0x9E3779B97F4A7C15ull ^ key);
+        const double u = (double)(h >> 11) * (1.0 / 9007199254740992.0);
+        E[lin] = (T)(v * (1.0 + noise * (2.0 * u - 1.0)));
+    }
+}
+
+template <typename T>
+hipError_t launch_init_separable(const Geo &g, const double *f, double peak, double noise, uint64_t key, T *E,
+                                 hipStream_t s)
+{
+    uint32_t nb = nblocks(g.ncells, 256);
+    nb = nb > 16384 ? 16384 : (nb ? nb : 1);
+    hipLaunchKernelGGL(k_init_separable<T>, dim3(nb), dim3(256), 0, s, g, f, peak, noise, key, E);
+    return hipGetLastError();
+}
+template hipError_t launch_init_separable<double>(const Geo &, const double *, double, double, uint64_t, double *,
+                                                  hipStream_t);
+template hipError_t launch_init_separable<float>(const Geo &, const double *, double, double, uint64_t, float *,
+                                                 hipStream_t);
 
 __global__ void k_d2f(const double *__restrict__ in, float *__restrict__ out, size_t n)
 {

@@ -567,10 +567,18 @@ inline uint32_t col_blocks(const Geo &g, const ColCfg &cc)
     return (uint32_t)((nthreads + 255) / 256);
 }
 
-// Ensemble step (rt6d_fast.hip): one persistent launch over the task list.
+// Ensemble step (rt6d_fast.hip): one launch per variant over its stretch of
+// the task list (tasks + off[v], cnt[v] tasks); persist_blocks = 0 runs one
+// block per task, else that many persistent blocks share the tasks through
+// the work counters ctr (kEnsCounters entries, cleared by the launch).
 template <typename T>
-hipError_t launch_ensemble(const EnsMember<T> *mem, const EnsTask *tasks, uint32_t ntasks, uint32_t parity,
-                           uint32_t nchunks, hipStream_t s);
+hipError_t launch_ensemble(const EnsMember<T> *mem, const EnsTask *tasks, const uint32_t off[kEnsVariants],
+                           const uint32_t cnt[kEnsVariants], uint32_t parity, uint32_t persist_blocks, uint32_t *ctr,
+                           hipStream_t s);
+
+// Superbee branch flips between a binary64 and a binary32 integration (rt6d_fast.hip).
+hipError_t launch_branch_flips(const Geo &g, const StepPtrs<double> &a, bool h64, const StepPtrs<float> &b, bool h32,
+                               unsigned long long *counts, hipStream_t s);
 
 // Implemented in rt6d_exact.hip (EXACT, double only) and rt6d_fast.hip.
 hipError_t launch_step_exact(const Geo &g, const StepPtrs<double> &p, bool homog, const LaunchCfg &lc);
@@ -618,6 +626,9 @@ hipError_t launch_vmax(const Geo &g, const double *const vel[kAxes], unsigned lo
 hipError_t launch_cfl_max(const Geo &g, const double *const vel[kAxes], const AxisCoef &ax,
                           unsigned long long *bits, hipStream_t s);
 hipError_t launch_convert_d2f(const double *in, float *out, size_t n, hipStream_t s);
+template <typename T>
+hipError_t launch_init_separable(const Geo &g, const double *f, double peak, double noise, uint64_t key, T *E,
+                                 hipStream_t s);
 hipError_t launch_copy(const void *in, void *out, size_t bytes, hipStream_t s);
 hipError_t launch_convert_f2d(const float *in, double *out, size_t n, hipStream_t s);
 

@@ -39,7 +39,8 @@ EXPORTS = ("rt6d_version", "rt6d_last_error", "rt6d_device_count", "rt6d_create"
            "rt6d_upload_E", "rt6d_step", "rt6d_step_timed", "rt6d_fluxes", "rt6d_download", "rt6d_diagnose",
            "rt6d_set_option", "rt6d_query", "rt6d_synchronize", "rt6d_copy_bandwidth",
            "rt6d_ensemble_create", "rt6d_ensemble_destroy", "rt6d_ensemble_add", "rt6d_ensemble_remove",
-           "rt6d_ensemble_step", "rt6d_ensemble_set_schedule", "rt6d_ensemble_tune", "rt6d_ensemble_info")
+           "rt6d_ensemble_step", "rt6d_ensemble_set_schedule", "rt6d_ensemble_tune", "rt6d_ensemble_info",
+           "rt6d_init_separable", "rt6d_branch_flips", "rt6d_ensemble_step_async", "rt6d_ensemble_wait")
 
 
 class Diag(C.Structure):
@@ -104,6 +105,10 @@ def load_library(path: str | None = None):
         "rt6d_ensemble_set_schedule": ([vp, C.c_int, C.c_int], C.c_int),
         "rt6d_ensemble_tune": ([vp, C.c_int, ip, ip], C.c_int),
         "rt6d_ensemble_info": ([vp, ip, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)], C.c_int),
+        "rt6d_init_separable": ([vp, dp, C.c_double, C.c_double, C.c_ulonglong, C.c_ulonglong], C.c_int),
+        "rt6d_branch_flips": ([vp, vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)], C.c_int),
+        "rt6d_ensemble_step_async": ([vp, C.c_int], C.c_int),
+        "rt6d_ensemble_wait": ([vp, dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -138,6 +143,14 @@ def device_copy_timed(nbytes: int, reps: int = 10) -> float:
     ms = C.c_double()
     _check(lib, lib.rt6d_copy_bandwidth(int(nbytes), reps, C.byref(ms)), "rt6d_copy_bandwidth")
     return ms.value
+
+
+def branch_flips(m64: "Model", m32: "Model"):
+    """(faces, flips) per axis between a binary64 and a binary32 model (rt6d_branch_flips)."""
+    lib = load_library()
+    faces, flips = (C.c_longlong * 6)(), (C.c_longlong * 6)()
+    _check(lib, lib.rt6d_branch_flips(m64._h, m32._h, faces, flips), "rt6d_branch_flips")
+    return list(faces), list(flips)
 
 
 def build_grid(axis: int, spacing, fixed: float = 0.0):
@@ -202,6 +215,14 @@ class Model:
         E = np.ascontiguousarray(E, dtype=np.float64).ravel()
         _check(self.lib, self.lib.rt6d_upload_E(self._h, _dptr(E), layout), "rt6d_upload_E")
 
+    def init_separable(self, factors, peak: float, noise: float, seed: int, member: int):
+        """E on the device: peak*fx*fy*fz*fk*fl*fm*(1+noise*(2u-1)) (rt6d_init_separable)."""
+        f = np.ascontiguousarray(np.concatenate([np.asarray(a, dtype=np.float64) for a in factors]))
+        if f.size != sum(self.n):
+            raise ValueError("factors must hold n0+...+n5 values")
+        _check(self.lib, self.lib.rt6d_init_separable(self._h, _dptr(f), peak, noise, seed, member),
+               "rt6d_init_separable")
+
     @classmethod
     def from_case(cls, case: "_cases.Case", precision=FP64, exact=False, path=0, block=None,
                   kernel=0) -> "Model":
@@ -217,7 +238,11 @@ class Model:
         for ax, (t, dt) in enumerate(case.grids()):
             m.set_grid(ax, t, dt)
         m.set_background(case.Nsqr, case.U, case.V, case.f0, case.sign)
-        m.upload_E(case.E0)
+        if case.E0 is None and "init" in case.meta:
+            i = case.meta["init"]
+            m.init_separable(i["factors"], i["peak"], i["noise"], i["seed"], i["member"])
+        else:
+            m.upload_E(case.E0)
         m.dt = case.dt
         return m
 
@@ -339,6 +364,16 @@ class Ensemble:
         """Advance every member nsteps steps; returns the device time in ms."""
         ms = C.c_double()
         _check(self.lib, self.lib.rt6d_ensemble_step(self._h, nsteps, C.byref(ms)), "rt6d_ensemble_step")
+        return ms.value
+
+    def step_async(self, nsteps: int = 1):
+        """Enqueue nsteps steps and return (see wait)."""
+        _check(self.lib, self.lib.rt6d_ensemble_step_async(self._h, nsteps), "rt6d_ensemble_step_async")
+
+    def wait(self) -> float:
+        """Wait for enqueued steps; returns their device time in ms."""
+        ms = C.c_double()
+        _check(self.lib, self.lib.rt6d_ensemble_wait(self._h, C.byref(ms)), "rt6d_ensemble_wait")
         return ms.value
 
     def set_schedule(self, blocks_per_cu: int, order: int = 0):

@@ -1,0 +1,376 @@
+"""Request loop with continuous batching of resident integrations.
+
+Requests (one integration each: grid, stratification, shear and initial
+packet drawn per member, a step count and a precision) arrive over time.  The
+server admits them, in arrival order, as soon as the HBM budget allows,
+builds each one on the device (grids, background, calc_parameter, E from the
+separable initializer, dt at CFL 0.5) on a set-up thread while the device
+steps the others, and advances all admitted integrations of one precision
+together in a persistent-kernel ensemble step (rt6d_ensemble_step, two steps
+per HIP graph replay).  Every tick advances each ensemble by
+min(chunk, steps left of its members); members whose steps are done leave
+their ensemble, their result (E, and the energy budget when asked) is read,
+their memory freed, and queued requests take the room.  State stays on the
+device from admission to completion.
+
+    python -m raytracer_3d_amd.serve --rate 40 --duration 120 --budget-gb 200
+
+prints one JSON line: p50/p99 completion latency (arrival to result), the
+aggregate cell-updates/s of the served run, and the same requests run one at
+a time with the plain binary64 step (``--baseline-seconds`` bounds that
+sample; its latency is the FIFO single-server queue of the measured service
+times under the same arrivals).
+
+Results do not depend on the company a request keeps: the ensemble runs the
+same column-kernel arithmetic as a single integration, so each result equals
+the one computed alone (checked bit for bit by the GPU tests).
+"""
+from __future__ import annotations
+
+import argparse
+import collections
+import dataclasses
+import json
+import math
+import queue
+import sys
+import threading
+import time
+
+import numpy as np
+
+from . import cases, rt6d
+
+TUNE_STEPS = 8 * (2 + 2)   # rt6d_ensemble_tune with trial_steps = 2: 8 schedules x 4 steps
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    arrival: float          # seconds after the start of the run
+    member: int             # member index of the generator
+    n: tuple
+    steps: int
+    precision: int          # rt6d.FP64 or rt6d.FP32
+    seed: int = 4242
+    # filled by the server
+    admitted: float = math.nan
+    started: float = math.nan
+    done: float = math.nan
+    result: dict | None = None
+
+    @property
+    def cells(self) -> int:
+        return int(np.prod(self.n))
+
+
+def request_stream(rate: float, duration: float, seed: int = 4242, steps_range=(50, 1000), fp32_share=0.75,
+                   n=None, first_member: int = 0):
+    """Poisson arrivals at `rate` per second for `duration` seconds; each
+    request is a member of the generator (grids from {12..24}^3 x {8..16}^3),
+    with steps ~ U[steps_range] and FP32 with probability `fp32_share`."""
+    out = []
+    t = 0.0
+    i = 0
+    while True:
+        u = cases.uniform(seed, first_member + i, np.arange(4, dtype=np.uint64), stream=11)
+        t += -math.log1p(-u[0]) / rate
+        if t >= duration:
+            break
+        spec_n = n if n is not None else cases.member_spec(seed, first_member + i).n
+        steps = int(steps_range[0] + (steps_range[1] - steps_range[0] + 1) * u[1])
+        steps = min(steps, steps_range[1])
+        prec = rt6d.FP32 if u[2] < fp32_share else rt6d.FP64
+        out.append(Request(i, t, first_member + i, tuple(spec_n), steps, prec, seed))
+        i += 1
+    return out
+
+
+def resident_bytes(n, precision) -> int:
+    """Device bytes of an integration after calc_parameter (E twice, the six
+    face-velocity fields, lambda, grids, coefficients, background)."""
+    cells = int(np.prod(n))
+    es = 8 if precision == rt6d.FP64 else 4
+    faces = sum(cells // n[d] * (n[d] + 1) for d in range(6))
+    n3h = (n[0] + 4) * (n[1] + 4) * (n[2] + 4)
+    return es * (3 * cells + faces) + 8 * (3 * n3h) + 8 * 5 * sum(v + 4 for v in n) + (1 << 16)
+
+
+def peak_bytes(n, precision) -> int:
+    """Device bytes while calc_parameter runs (binary64 fields first)."""
+    cells = int(np.prod(n))
+    faces = sum(cells // n[d] * (n[d] + 1) for d in range(6))
+    extra = 0 if precision == rt6d.FP64 else 8 * (faces + cells)
+    return resident_bytes(n, precision) + extra
+
+
+def build_member(req: Request, cfl: float = 0.5):
+    """Device-resident integration of a request (calc_parameter done, E set)."""
+    spec = cases.member_spec(req.seed, req.member, n=req.n)
+    m = rt6d.Model.from_case(spec, precision=req.precision)
+    m.calc_parameter()
+    m.dt = m.cfl_dt(cfl)
+    return m
+
+
+class Server:
+    """Continuous batching over one ensemble per precision."""
+
+    def __init__(self, budget_bytes: float, chunk: int = 8, max_members: int = 1024, keep_E: bool = False,
+                 diagnose: bool = False, tune: bool = True, tune_min_members: int = 8):
+        self.budget = float(budget_bytes)
+        self.chunk = int(chunk)
+        self.max_members = int(max_members)
+        self.keep_E = keep_E
+        self.diagnose = diagnose
+        self.tune = tune
+        self.tune_min_members = int(tune_min_members)
+        self.ens = {rt6d.FP64: rt6d.Ensemble(rt6d.FP64), rt6d.FP32: rt6d.Ensemble(rt6d.FP32)}
+        self.tuned = {rt6d.FP64: False, rt6d.FP32: False}
+        self.used = 0.0
+        self.peak_used = 0.0
+        self.ticks = 0
+        self.device_ms = 0.0
+
+    def close(self):
+        for e in self.ens.values():
+            e.close()
+
+    def run(self, requests, log=None):
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        pending = collections.deque(reqs)
+        waiting = collections.deque()
+        ready = queue.Queue()
+        active = {rt6d.FP64: {}, rt6d.FP32: {}}   # slot -> (req, model, steps left)
+        building = 0
+        lock = threading.Lock()
+        jobs = queue.Queue()
+
+        def builder():
+            while True:
+                r = jobs.get()
+                if r is None:
+                    return
+                try:
+                    m = build_member(r)
+                    ready.put((r, m, None))
+                except Exception as e:  # noqa: BLE001 -- reported to the loop
+                    ready.put((r, None, e))
+
+        th = threading.Thread(target=builder, daemon=True)
+        th.start()
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0  # noqa: E731
+        done = 0
+        total = len(reqs)
+        try:
+            while done < total:
+                t = now()
+                while pending and pending[0].arrival <= t:
+                    waiting.append(pending.popleft())
+                # admission in arrival order against the HBM budget
+                while waiting and building + sum(len(a) for a in active.values()) < self.max_members:
+                    r = waiting[0]
+                    need = peak_bytes(r.n, r.precision)
+                    with lock:
+                        if self.used + need > self.budget and (building or any(active.values())):
+                            break
+                        self.used += need
+                        self.peak_used = max(self.peak_used, self.used)
+                    waiting.popleft()
+                    r.admitted = now()
+                    building += 1
+                    jobs.put(r)
+                # members built since the last tick join their ensemble
+                while True:
+                    try:
+                        r, m, err = ready.get_nowait()
+                    except queue.Empty:
+                        break
+                    building -= 1
+                    if err is not None:
+                        raise err
+                    with lock:
+                        self.used += resident_bytes(r.n, r.precision) - peak_bytes(r.n, r.precision)
+                    slot = self.ens[r.precision].add(m)
+                    r.started = now()
+                    active[r.precision][slot] = [r, m, r.steps]
+                if not any(active.values()):
+                    if building:
+                        r, m, err = ready.get()   # wait for the set-up thread
+                        ready.put((r, m, err))
+                        continue
+                    if pending:
+                        time.sleep(max(0.0, min(0.01, pending[0].arrival - now())))
+                    continue
+                launched = []
+                for prec, act in active.items():
+                    if not act:
+                        continue
+                    e = self.ens[prec]
+                    if self.tune and not self.tuned[prec] and len(act) >= self.tune_min_members:
+                        left = min(v[2] for v in act.values())
+                        if left >= TUNE_STEPS + self.chunk:
+                            e.tune(trial_steps=2)   # counted in the members' steps
+                            for v in act.values():
+                                v[2] -= TUNE_STEPS
+                            self.tuned[prec] = True
+                    s = min(self.chunk, min(v[2] for v in act.values()))
+                    e.step_async(s)    # both precisions' ensembles run on the device together
+                    launched.append((prec, s))
+                for prec, s in launched:
+                    e = self.ens[prec]
+                    act = active[prec]
+                    self.device_ms += e.wait()
+                    self.ticks += 1
+                    finished = []
+                    for slot, v in act.items():
+                        v[2] -= s
+                        if v[2] == 0:
+                            finished.append(slot)
+                    for slot in finished:
+                        r, m, _ = act.pop(slot)
+                        e.remove(slot)
+                        res = {"steps": m.query(rt6d.Q_STEPS), "dt": m.dt}
+                        if self.diagnose:
+                            d = m.diagnose()
+                            res.update(eint=d["eint"], residual=d["residual"])
+                        if self.keep_E:
+                            res["E"] = m.download("E")
+                        r.result = res
+                        m.close()
+                        r.done = now()
+                        with lock:
+                            self.used -= resident_bytes(r.n, r.precision)
+                        done += 1
+                        if log and done % 50 == 0:
+                            log(f"served {done}/{total} at t={r.done:.1f}s, resident "
+                                f"{sum(len(a) for a in active.values())}, used {self.used / 2**30:.1f} GiB")
+        finally:
+            jobs.put(None)
+            th.join(timeout=30)
+        return now()
+
+
+def run_alone(req: Request, precision=None):
+    """The request run on its own (plain single-integration steps); returns
+    (seconds, E)."""
+    r = dataclasses.replace(req, precision=req.precision if precision is None else precision)
+    m = build_member(r)
+    m.synchronize()
+    t0 = time.perf_counter()
+    m.integrate(r.steps)
+    t = time.perf_counter() - t0
+    E = m.download("E")
+    m.close()
+    return t, E
+
+
+def fifo_latencies(arrivals, service):
+    """Completion latencies of a FIFO single server."""
+    free = 0.0
+    out = []
+    for a, s in zip(arrivals, service):
+        start = max(a, free)
+        free = start + s
+        out.append(free - a)
+    return out
+
+
+def serve_benchmark(rate=40.0, duration=120.0, budget_gb=200.0, chunk=8, baseline_seconds=20.0, seed=4242,
+                    log=None):
+    reqs = request_stream(rate, duration, seed=seed)
+    work = sum(r.cells * r.steps for r in reqs)
+    srv = Server(budget_gb * 2**30, chunk=chunk)
+    try:
+        wall = srv.run(reqs, log=log)
+    finally:
+        srv.close()
+    lat = np.array([r.done - r.arrival for r in reqs])
+    served = {
+        "requests": len(reqs), "cell_updates": work, "wall_s": wall, "value": work / wall,
+        "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+        "mean_latency_s": float(lat.mean()), "device_ms": srv.device_ms, "ticks": srv.ticks,
+        "peak_hbm_gib": srv.peak_used / 2**30,
+        "fp32_share": float(np.mean([r.precision == rt6d.FP32 for r in reqs])),
+    }
+    # the same requests one at a time with the plain binary64 step (bounded sample)
+    svc, arr, wk = [], [], 0
+    t_start = time.perf_counter()
+    for r in reqs:
+        if time.perf_counter() - t_start > baseline_seconds:
+            break
+        t, _ = run_alone(r, precision=rt6d.FP64)
+        svc.append(t)
+        arr.append(r.arrival)
+        wk += r.cells * r.steps
+    base_rate = wk / sum(svc) if svc else None
+    # the whole stream served one at a time at the sampled per-request rate
+    est = [r.cells * r.steps / base_rate for r in reqs] if base_rate else []
+    blat = np.array(fifo_latencies([r.arrival for r in reqs], est)) if est else np.array([math.nan])
+    baseline = {
+        "kind": "one at a time, plain binary64 step (rt6d_step), set-up excluded",
+        "sampled_requests": len(svc), "value": base_rate,
+        "p50_latency_s": float(np.percentile(blat, 50)), "p99_latency_s": float(np.percentile(blat, 99)),
+    }
+    served["speedup_vs_one_at_a_time_fp64"] = (served["value"] / base_rate) if base_rate else None
+    return {"served": served, "one_at_a_time_fp64": baseline,
+            "scenario": {"rate_per_s": rate, "duration_s": duration, "budget_gib": budget_gb, "chunk": chunk,
+                         "steps": "U[50,1000]", "fp32:fp64": "3:1", "members": "{12,16,20,24}^3 x {8,12,16}^3"}}
+
+
+def ensemble_benchmark(members=96, steps=300, precision=rt6d.FP64, seed=4242, alone_members=None, log=None):
+    """Batched ensemble: `members` generator members in one ensemble,
+    autotuned, then `steps` timed steps; against the same members stepped
+    one at a time (a sample of `alone_members`, all if None)."""
+    reqs = [Request(i, 0.0, i, cases.member_spec(seed, i).n, steps, precision, seed) for i in range(members)]
+    t0 = time.perf_counter()
+    ms = [build_member(r) for r in reqs]
+    setup = time.perf_counter() - t0
+    cells = sum(r.cells for r in reqs)
+    with rt6d.Ensemble(precision) as ens:
+        for m in ms:
+            ens.add(m)
+        bpc, order = ens.tune(trial_steps=2)
+        ens.step(2)
+        t1 = time.perf_counter()
+        dev_ms = ens.step(steps)
+        wall = time.perf_counter() - t1
+    for m in ms:
+        m.close()
+    sample = reqs if alone_members is None else reqs[:alone_members]
+    t_alone, w_alone = 0.0, 0
+    for r in sample:
+        m = build_member(r)
+        m.integrate(2)
+        m.synchronize()
+        t_alone += m.integrate_timed(steps) * 1e-3
+        w_alone += r.cells * steps
+        m.close()
+    out = {"members": members, "steps": steps, "cells": cells, "setup_s": setup, "schedule": [bpc, order],
+           "wall_s": wall, "device_ms": dev_ms, "value": cells * steps / wall,
+           "one_at_a_time": w_alone / t_alone if t_alone else None, "one_at_a_time_members": len(sample)}
+    out["speedup_vs_one_at_a_time"] = out["value"] / out["one_at_a_time"] if out["one_at_a_time"] else None
+    if log:
+        log(f"ensemble {members} members x {steps} steps ({'fp32' if precision else 'fp64'}): "
+            f"{out['value']:.3e} cell-updates/s, one at a time {out['one_at_a_time']:.3e}")
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--rate", type=float, default=40.0)
+    ap.add_argument("--duration", type=float, default=120.0)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--chunk", type=int, default=8)
+    ap.add_argument("--baseline-seconds", type=float, default=20.0)
+    ap.add_argument("--seed", type=int, default=4242)
+    a = ap.parse_args(argv)
+    out = serve_benchmark(a.rate, a.duration, a.budget_gb, a.chunk, a.baseline_seconds, a.seed,
+                          log=lambda s: print("[serve] " + s, file=sys.stderr, flush=True))
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

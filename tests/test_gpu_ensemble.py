@@ -61,7 +61,11 @@ def test_ensemble_equals_individual_runs(precision, gpu_available):
         ens.step(5)
         ens.set_schedule(8, 0)
         ens.step(2)
-        for c, m, n in ((cs[0], ms[0], 15), (cs[2], ms[2], 15), (cs[3], ms[3], 15), (new_case, new, 7)):
+        ens.set_schedule(0, 1)      # one block per task
+        ens.step_async(3)
+        ens.step_async(2)           # enqueued behind the first
+        assert ens.wait() > 0.0
+        for c, m, n in ((cs[0], ms[0], 20), (cs[2], ms[2], 20), (cs[3], ms[3], 20), (new_case, new, 12)):
             E = m.download("E")
             ref = _alone(c, precision, n, path=1 if c is cs[3] else 0)
             assert np.array_equal(E, ref), f"{c.name}: max diff {np.max(np.abs(E - ref))}"
@@ -79,9 +83,9 @@ def test_ensemble_tune_and_state(gpu_available):
         for m in ms:
             ens.add(m)
         bpc, order = ens.tune(trial_steps=2)
-        assert bpc in (2, 4, 8) and order in (0, 1)
+        assert bpc in (0, 2, 4, 8) and order in (0, 1)
         steps = ens.info()["steps"]
-        assert steps == 6 * (2 + 2)
+        assert steps == 8 * (2 + 2)
         ens.step(1)
         for c, m in zip(cs, ms):
             assert np.array_equal(m.download("E"), _alone(c, rt6d.FP64, steps + 1))

@@ -1,0 +1,57 @@
+"""Host logic of the serving loop (no device): the request stream, the FIFO
+queue model and the memory estimates."""
+import math
+
+import numpy as np
+
+from raytracer_3d_amd import cases, rt6d, serve
+
+
+def test_request_stream_is_seeded_poisson_with_the_stated_mix():
+    a = serve.request_stream(40.0, 30.0, seed=5)
+    b = serve.request_stream(40.0, 30.0, seed=5)
+    assert [(r.arrival, r.n, r.steps, r.precision) for r in a] == [(r.arrival, r.n, r.steps, r.precision) for r in b]
+    assert 900 < len(a) < 1500                      # 1200 expected
+    gaps = np.diff([0.0] + [r.arrival for r in a])
+    assert abs(gaps.mean() - 1 / 40.0) < 0.1 / 40.0   # exponential mean 1/rate
+    steps = np.array([r.steps for r in a])
+    assert steps.min() >= 50 and steps.max() <= 1000 and abs(steps.mean() - 525) < 40
+    share = np.mean([r.precision == rt6d.FP32 for r in a])
+    assert abs(share - 0.75) < 0.05
+    for r in a[:50]:
+        assert r.n[0] in (12, 16, 20, 24) and r.n[3] in (8, 12, 16)
+
+
+def test_fifo_latencies_closed_form():
+    # one server, unit service: arrivals at 0, 0, 0 finish at 1, 2, 3
+    assert serve.fifo_latencies([0, 0, 0], [1, 1, 1]) == [1, 2, 3]
+    # idle gaps: no waiting
+    assert serve.fifo_latencies([0, 5, 10], [1, 1, 1]) == [1, 1, 1]
+
+
+def test_memory_estimates_scale_with_cells_and_precision():
+    n = (24, 24, 24, 16, 16, 16)
+    r64, r32 = serve.resident_bytes(n, rt6d.FP64), serve.resident_bytes(n, rt6d.FP32)
+    cells = math.prod(n)
+    assert 9 * 8 * cells < r64 < 10 * 8 * cells
+    assert abs(r64 / r32 - 2.0) < 0.01
+    assert serve.peak_bytes(n, rt6d.FP32) > r32 and serve.peak_bytes(n, rt6d.FP64) == r64
+
+
+def test_separable_E_product_order():
+    f = [np.array([1.0, 2.0]), np.array([3.0]), np.array([0.5, 0.25]), np.array([1.0]), np.array([1.0]),
+         np.array([7.0])]
+    E = cases.separable_E(f, 2.0, 0.0, 1, 1)
+    # Fortran order, x fastest: E[x + 2*z] = 2*fx*3*fz*7
+    assert E.tolist() == [2 * 1 * 3 * 0.5 * 7, 2 * 2 * 3 * 0.5 * 7, 2 * 1 * 3 * 0.25 * 7, 2 * 2 * 3 * 0.25 * 7]
+    g = cases.separable_E(f, 2.0, 0.01, 1, 1)
+    u = cases.uniform(1, 1, np.arange(4, dtype=np.uint64))
+    assert np.array_equal(g, E * (1.0 + 0.01 * (2.0 * u - 1.0)))
+
+
+def test_member_spec_draws_the_member_grid():
+    for m in range(20):
+        s = cases.member_spec(4242, m)
+        assert s.E0 is None and len(s.meta["init"]["factors"]) == 6
+        assert [len(f) for f in s.meta["init"]["factors"]] == list(s.n)
+        assert 2e-3 <= s.meta["N0"] <= 8e-3 and 0.0 <= s.meta["U0"] <= 0.3
