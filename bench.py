@@ -136,8 +136,8 @@ def _step_kernel(precision, homog, nz):
     general path)."""
     zc = 2 if nz % 2 == 0 else 1
     h = "true" if homog else "false"
-    ldsy = "false" if homog else "true"
-    return f"k_step_col<{precision}, false, {h}, {zc}, true, {ldsy}>"
+    ym = 0 if homog else 1
+    return f"k_step_col<{precision}, false, {h}, {zc}, true, {ym}>"
 
 
 def main():

@@ -71,7 +71,9 @@ extern "C" {
                                 7/8 plane kernel with x/y face-flux exchange in LDS,
                                 1/2 z cells per thread (nx*ny <= 1024);
                                 9/10/11 column kernel with the y neighbours staged in
-                                LDS, 1/2/4 z cells per thread */
+                                LDS, 1/2/4 z cells per thread;
+                                12/13/14 the same with one block per (x,y) plane set, every
+                                y neighbour from LDS (nx*ny <= 1024) */
 
 /* queries for rt6d_query */
 #define RT6D_Q_HOMOGENEOUS 1 /* 1 if the horizontally homogeneous path is active */
@@ -157,6 +159,13 @@ int rt6d_init_separable(rt6d_ctx *ctx, const double *factors, double peak, doubl
  * own current state in its own arithmetic; faces[d] counts the faces
  * compared, flips[d] those whose branches differ. */
 int rt6d_branch_flips(rt6d_ctx *fp64_ctx, rt6d_ctx *fp32_ctx, long long faces[6], long long flips[6]);
+
+/* set_forcing on the device, applied to E before every step (for_src/main.f90:69
+ * calls set_forcing before integrate).  kind 0 none; 1 bottom flux injection,
+ * E(z=1) += dt*field/dzt(1) (for_config/lee1.f90:199-212); 2 bottom relaxation,
+ * E(z=1) -= dt*max(0,zdot)*(E(z=1)-field)/dzt(1) (for_config/critical_gm1.f90:125-139).
+ * field: 5D over (x,y,k,l,m), nx*ny*nk*nl*nm values, Fortran order; copied. */
+int rt6d_set_forcing(rt6d_ctx *ctx, int kind, const double *field);
 
 /* integrate (for_src/integrate.f90:4-271) called nsteps times with step dt. */
 int rt6d_step(rt6d_ctx *ctx, double dt, int nsteps);

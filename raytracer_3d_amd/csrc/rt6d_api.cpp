@@ -100,6 +100,8 @@ struct rt6d_ctx {
     int *d_flag = nullptr;
     unsigned long long *d_bits = nullptr;
     rt6d_ensemble *owner = nullptr;  // ensemble this context is a member of
+    int forcing = 0;                 // bottom forcing applied before each step (0 none)
+    double *d_force = nullptr;       // its 5D field over (x,y,k,l,m)
 
     int dalloc(void **p, size_t bytes)
     {
@@ -472,6 +474,7 @@ int rt6d_destroy(rt6d_ctx *c)
     if (c->owner) ens_detach(c->owner, c);
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
+    c->dfree((void *&)c->d_force);
     c->dfree((void *&)c->d_grid);
     c->dfree((void *&)c->d_coef);
     c->dfree((void *&)c->d_bg);
@@ -720,6 +723,26 @@ int rt6d_init_separable(rt6d_ctx *c, const double *factors, double peak, double 
     return 0;
 }
 
+int rt6d_set_forcing(rt6d_ctx *c, int kind, const double *field)
+{
+    int r;
+    if ((r = check_ctx(c))) return r;
+    if ((r = not_member(c, "rt6d_set_forcing"))) return r;
+    if (kind < 0 || kind > 2) return fail(RT6D_EINVAL, "rt6d_set_forcing: kind %d", kind);
+    if (kind == 0) {
+        c->forcing = 0;
+        return 0;
+    }
+    if (!field) return fail(RT6D_EINVAL, "rt6d_set_forcing: null field");
+    if (c->n[2] < 1) return fail(RT6D_EINVAL, "rt6d_set_forcing: no z axis");
+    const size_t n5 = c->cells / (size_t)c->n[2];
+    if (!c->d_force && (r = c->dalloc((void **)&c->d_force, n5 * sizeof(double)))) return r;
+    HIPCHK(hipMemcpyAsync(c->d_force, field, n5 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->forcing = kind;
+    return 0;
+}
+
 int rt6d_branch_flips(rt6d_ctx *a, rt6d_ctx *b, long long faces[6], long long flips[6])
 {
     int r;
@@ -761,9 +784,18 @@ static int do_steps(rt6d_ctx *c, double dt, int nsteps)
     if (nsteps == 0) return 0;
     if ((r = ensure_coef(c, dt))) return r;
     const LaunchCfg lc{c->opt_block, c->opt_kernel, c->stream};
+    const double dz1 = c->dt_h[2][2];   // dzt(1): halo slot f+1 holds Fortran index f
     for (int s = 0; s < nsteps; ++s) {
         void *in = c->d_E[c->cur], *out = c->d_E[c->cur ^ 1];
         hipError_t e;
+        if (c->forcing) {   // set_forcing precedes integrate (main.f90:69-70)
+            e = c->prec == RT6D_FP64
+                    ? launch_bottom_forcing<double>(c->g, c->forcing, (double *)in, c->d_force,
+                                                    (const double *)c->d_vel[2], dt, dz1, c->stream)
+                    : launch_bottom_forcing<float>(c->g, c->forcing, (float *)in, c->d_force,
+                                                   (const float *)c->d_vel[2], dt, dz1, c->stream);
+            HIPCHK(e);
+        }
         if (c->prec == RT6D_FP64) {
             StepPtrs<double> p = step_ptrs<double>(c, in, out, dt);
             e = c->opt_exact ? launch_step_exact(c->g, p, c->homog, lc)
@@ -1019,7 +1051,14 @@ int rt6d_set_option(rt6d_ctx *c, int option, int value)
         c->opt_block = value;
         return 0;
     case RT6D_OPT_KERNEL:
-        if (value < 0 || value > 11) return fail(RT6D_EINVAL, "kernel option %d", value);
+        if (value < 0 || value > 14) return fail(RT6D_EINVAL, "kernel option %d", value);
+        if (value >= 12) {
+            const int zc = value == 12 ? 1 : (value == 13 ? 2 : 4);
+            const bool fits = c->prec == RT6D_FP64 ? colp_fits<double>(c->g, zc) : colp_fits<float>(c->g, zc);
+            if (!fits)
+                return fail(RT6D_EINVAL, "plane-block column kernel needs nx*ny <= 1024, nz divisible by %d and "
+                            "its LDS buffers within 64 KiB (have nx*ny = %d)", zc, c->n[0] * c->n[1]);
+        }
         if (value == 7 || value == 8) {
             const int zc = value == 8 ? 2 : 1;
             const bool fits = c->prec == RT6D_FP64 ? plane_fits<double>(c->g, zc) : plane_fits<float>(c->g, zc);
@@ -1335,6 +1374,7 @@ int rt6d_ensemble_add(rt6d_ensemble *e, rt6d_ctx *c, double dt, int *slot)
     if ((r = require_ready(c))) return r;
     if (c->owner) return fail(RT6D_ESTATE, "rt6d_ensemble_add: context already in an ensemble");
     if (c->opt_exact) return fail(RT6D_EINVAL, "rt6d_ensemble_add: ensembles run the fast kernels (exact option set)");
+    if (c->forcing) return fail(RT6D_EINVAL, "rt6d_ensemble_add: ensembles do not apply forcing (set_forcing kind 0 first)");
     ColCfg cc;
     int zc;
     const bool fits = e->prec == RT6D_FP64 ? col_config<double>(c->g, cc, zc) : col_config<float>(c->g, cc, zc);

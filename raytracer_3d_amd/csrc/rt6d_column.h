@@ -9,7 +9,7 @@
 //  * the ZC cells share their ZC-1 inner z faces; the four outer z
 //    neighbours are read one plane ahead;
 //  * x and y neighbours are plain loads (L1/L2 hits: the neighbouring lanes
-//    read the same lines), or, with LDSY, the y neighbours come from the
+//    read the same lines), or (YM != 0) the y neighbours come from the
 //    block's planes staged in LDS; both faces are computed per cell;
 //  * the (x,y) index is padded to a multiple of 64 so that a wavefront never
 //    spans two (z,k,l) planes.  Every access is then a raw buffer access with
@@ -82,12 +82,14 @@ template <bool HOMOG>
 inline bool is_full(const Geo &g) { return (g.act & full_mask<HOMOG>()) == full_mask<HOMOG>() && (HOMOG || g.act == 63); }
 
 // The work of logical block b (256 threads) of one integration.
-// LDSY: the y neighbours come from the block's copy of its (x,y) planes in
-// LDS (one barrier per m step, two buffers alternating with m) where the
-// neighbour lies inside the block's 256 positions, else from memory.  Lanes
-// beyond the grid then stay to the end (for the barriers) on clamped
-// in-bounds coordinates and store nothing.
-template <typename T, bool EXACT, bool HOMOG, int ZC, bool FULL = false, bool LDSY = false>
+// YM selects where the y neighbours come from: 0 memory; 1 the block's copy
+// of its 256 (x,y) positions in LDS (one barrier per m step, two buffers
+// alternating with m) where the neighbour lies inside the block, else
+// memory; 2 the same with one block per (x,y) plane set (blockDim = the
+// padded plane size, dynamic LDS), so that every y neighbour is in LDS.
+// With YM != 0 lanes beyond the grid stay to the end (for the barriers) on
+// clamped in-bounds coordinates and store nothing.
+template <typename T, bool EXACT, bool HOMOG, int ZC, bool FULL = false, int YM = 0>
 __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, const ColCfg &cc, uint32_t b)
 {
     const uint32_t t = b * blockDim.x + threadIdx.x;
@@ -101,7 +103,7 @@ __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, con
     int bl = (int)fdiv(kl, g.div[3]);
     int bk = (int)kl - bl * nk;
     bool live = true;
-    if constexpr (LDSY) {
+    if constexpr (YM != 0) {
         if (bl >= nl) { bl = nl - 1; bk = nk - 1; zb = cc.nzb - 1; live = false; }
         if (xy >= nxy) { xy = nxy - 1; live = false; }
     } else {
@@ -132,7 +134,7 @@ __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, con
     // per-thread byte offsets inside an (x,y) plane, fixed for the march
     const uint32_t oc = (uint32_t)xy * S;
     uint32_t ox[4], oy[4];
-    int ly[4];      // LDSY: block-local index of the y neighbour (own index when outside the block)
+    int ly[4];      // YM != 0: block-local index of the y neighbour (own index when outside the block)
     bool yin[4];
     {
         const int xs[4] = {x - 2 < 1 ? 1 : x - 2, x - 1 < 1 ? 1 : x - 1, x + 1 > nx ? nx : x + 1,
@@ -237,11 +239,18 @@ __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, con
         for (int q = 0; q < 4; ++q) znx[q] = (az && m < nm) ? ldz(q, m + 1) : T(0);
 
         T *sb = nullptr;
-        if constexpr (LDSY) {
-            __shared__ T sEy[2][ZC][256];
+        constexpr int NT1 = 256;
+        const int NT = YM == 2 ? (int)blockDim.x : NT1;
+        if constexpr (YM == 1) {
+            __shared__ T sEy[2][ZC][NT1];
             sb = &sEy[m & 1][0][0];
+        } else if constexpr (YM == 2) {
+            extern __shared__ unsigned char col_smem[];
+            sb = reinterpret_cast<T *>(col_smem) + (size_t)(m & 1) * ZC * NT;
+        }
+        if constexpr (YM != 0) {
 #pragma unroll
-            for (int j = 0; j < ZC; ++j) sb[j * 256 + threadIdx.x] = w0[j];
+            for (int j = 0; j < ZC; ++j) sb[j * NT + threadIdx.x] = w0[j];
             __syncthreads();
         }
         T Fz[ZC + 1];
@@ -288,13 +297,13 @@ __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, con
             }
             if (ay) {
                 T a0, a1, a3, a4;
-                if constexpr (LDSY) {
-                    const T *sj = sb + j * 256;
+                if constexpr (YM != 0) {
+                    const T *sj = sb + j * NT;
                     a0 = sj[ly[0]];
                     a1 = sj[ly[1]];
                     a3 = sj[ly[2]];
                     a4 = sj[ly[3]];
-                    if (!(yin[0] && yin[1] && yin[2] && yin[3])) {   // rows next to the block's ends
+                    if (YM == 1 && !(yin[0] && yin[1] && yin[2] && yin[3])) {   // rows next to the block's ends
                         if (!yin[0]) a0 = bld<T>(rE, oy[0], pb);
                         if (!yin[1]) a1 = bld<T>(rE, oy[1], pb);
                         if (!yin[2]) a3 = bld<T>(rE, oy[2], pb);
@@ -360,7 +369,7 @@ __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, con
             T enew;
             if constexpr (EXACT) enew = e0 + p.dt * dE;
             else enew = e0 + (T)p.dt * dE;
-            if (!LDSY || live) bst<T>(enew, rO, oc, pb);
+            if (YM == 0 || live) bst<T>(enew, rO, oc, pb);
         }
 #pragma unroll
         for (int j = 0; j < ZC; ++j) {
@@ -374,10 +383,20 @@ __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, con
     }
 }
 
-template <typename T, bool EXACT, bool HOMOG, int ZC, bool FULL, bool LDSY = false>
+template <typename T, bool EXACT, bool HOMOG, int ZC, bool FULL, int YM = 0>
 __global__ void __launch_bounds__(256) k_step_col(const Geo g, const StepPtrs<T> p, const ColCfg cc, uint32_t nbl)
 {
     const uint32_t b = xcd_logical_block(blockIdx.x, nbl);
     if (b >= nbl) return;   // uniform over the block
-    col_body<T, EXACT, HOMOG, ZC, FULL, LDSY>(g, p, cc, b);
+    col_body<T, EXACT, HOMOG, ZC, FULL, YM>(g, p, cc, b);
+}
+
+// One block per (x,y) plane set: blockDim = the padded plane size (<= 1024),
+// the y neighbours all from LDS (YM = 2).
+template <typename T, bool EXACT, bool HOMOG, int ZC, bool FULL>
+__global__ void __launch_bounds__(1024) k_step_colp(const Geo g, const StepPtrs<T> p, const ColCfg cc, uint32_t nbl)
+{
+    const uint32_t b = xcd_logical_block(blockIdx.x, nbl);
+    if (b >= nbl) return;   // uniform over the block
+    col_body<T, EXACT, HOMOG, ZC, FULL, 2>(g, p, cc, b);
 }

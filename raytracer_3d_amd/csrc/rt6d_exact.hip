@@ -484,6 +484,55 @@ template hipError_t launch_init_separable<double>(const Geo &, const double *, d
 template hipError_t launch_init_separable<float>(const Geo &, const double *, double, double, uint64_t, float *,
                                                  hipStream_t);
 
+// Forcing of the bottom plane z = 1 applied before each step, the
+// set_forcing hooks of the reference's config plug-ins:
+//   kind 1, bottom flux injection (for_config/lee1.f90:199-212):
+//     E = E + dt*F/dzt(1)
+//   kind 2, bottom relaxation towards F where the group velocity points up
+//     (for_config/critical_gm1.f90:125-139): E = E - dt*max(0,zdot)*(E-F)/dzt(1)
+// F is 5D over (x,y,k,l,m); zdot at the T point z = 1 is the face array's
+// entry 1.  Left-to-right binary64 evaluation as in the Fortran statements,
+// then rounded to the context's precision.
+template <typename T>
+__global__ void __launch_bounds__(256) k_bottom_forcing(const Geo g, int kind, T *__restrict__ E,
+                                                        const double *__restrict__ F, const T *__restrict__ zdot,
+                                                        double dt, double dz1)
+{
+#pragma clang fp contract(off)
+    const uint32_t nxy = (uint32_t)g.n[0] * g.n[1], nz = (uint32_t)g.n[2], nk = (uint32_t)g.n[3],
+                   nl = (uint32_t)g.n[4];
+    const uint32_t n5 = (uint32_t)(g.ncells / nz);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n5; i += gridDim.x * blockDim.x) {
+        const uint32_t xy = i % nxy, r = i / nxy;          // r = k + nk*(l + nl*m)
+        const uint32_t lin = xy + nxy * nz * r;            // cell (x, y, z = 1, k, l, m)
+        const double e = (double)E[lin];
+        if (kind == 1) {
+            E[lin] = (T)(e + dt * F[i] / dz1);
+        } else {
+            const uint32_t fz = xy + nxy * (1u + (nz + 1u) * r);   // z face 1 = T point z = 1
+            const double w = (double)zdot[fz];
+            E[lin] = (T)(e - dt * (w > 0.0 ? w : 0.0) * (e - F[i]) / dz1);
+        }
+    }
+    (void)nk;
+    (void)nl;
+}
+
+template <typename T>
+hipError_t launch_bottom_forcing(const Geo &g, int kind, T *E, const double *F, const T *zdot, double dt, double dz1,
+                                 hipStream_t s)
+{
+    const uint64_t n5 = g.ncells / (uint64_t)g.n[2];
+    uint32_t nb = nblocks(n5, 256);
+    nb = nb > 8192 ? 8192 : (nb ? nb : 1);
+    hipLaunchKernelGGL(k_bottom_forcing<T>, dim3(nb), dim3(256), 0, s, g, kind, E, F, zdot, dt, dz1);
+    return hipGetLastError();
+}
+template hipError_t launch_bottom_forcing<double>(const Geo &, int, double *, const double *, const double *, double,
+                                                  double, hipStream_t);
+template hipError_t launch_bottom_forcing<float>(const Geo &, int, float *, const double *, const float *, double,
+                                                 double, hipStream_t);
+
 __global__ void k_d2f(const double *__restrict__ in, float *__restrict__ out, size_t n)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)

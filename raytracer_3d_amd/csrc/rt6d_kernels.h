@@ -466,7 +466,7 @@ hipError_t launch_plane(const Geo &g, const StepPtrs<T> &p, bool homog, hipStrea
     return hipGetLastError();
 }
 
-template <typename T, bool EXACT, int ZC, bool LDSY = false>
+template <typename T, bool EXACT, int ZC, int YM = 0>
 hipError_t launch_col(const Geo &g, const StepPtrs<T> &p, bool homog, hipStream_t stream)
 {
     const uint32_t nxy = (uint32_t)g.n[0] * g.n[1];
@@ -480,14 +480,51 @@ hipError_t launch_col(const Geo &g, const StepPtrs<T> &p, bool homog, hipStream_
     const uint32_t nbh = ((nbl + 7u) / 8u) * 8u;
     if (homog) {
         if (is_full<true>(g))
-            hipLaunchKernelGGL((k_step_col<T, EXACT, true, ZC, true, LDSY>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
+            hipLaunchKernelGGL((k_step_col<T, EXACT, true, ZC, true, YM>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
         else
-            hipLaunchKernelGGL((k_step_col<T, EXACT, true, ZC, false, LDSY>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
+            hipLaunchKernelGGL((k_step_col<T, EXACT, true, ZC, false, YM>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
     } else {
         if (is_full<false>(g))
-            hipLaunchKernelGGL((k_step_col<T, EXACT, false, ZC, true, LDSY>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
+            hipLaunchKernelGGL((k_step_col<T, EXACT, false, ZC, true, YM>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
         else
-            hipLaunchKernelGGL((k_step_col<T, EXACT, false, ZC, false, LDSY>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
+            hipLaunchKernelGGL((k_step_col<T, EXACT, false, ZC, false, YM>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
+    }
+    return hipGetLastError();
+}
+
+// One block per (x,y) plane set: the padded plane fits one block and the
+// two LDS buffers of ZC planes fit 64 KiB.
+template <typename T>
+inline bool colp_fits(const Geo &g, int zc)
+{
+    const uint32_t nxy = (uint32_t)g.n[0] * g.n[1];
+    const uint32_t nthr = ((nxy + 63u) / 64u) * 64u;
+    return nthr <= 1024 && g.n[2] % zc == 0 && 2ull * zc * nthr * sizeof(T) <= 65536ull;
+}
+
+template <typename T, bool EXACT, int ZC>
+hipError_t launch_colp(const Geo &g, const StepPtrs<T> &p, bool homog, hipStream_t stream)
+{
+    const uint32_t nxy = (uint32_t)g.n[0] * g.n[1];
+    ColCfg cc;
+    cc.nxyp = (int)(((nxy + 63u) / 64u) * 64u);
+    cc.nzb = g.n[2] / ZC;
+    cc.dxyp = make_fastdiv((uint32_t)cc.nxyp);
+    cc.dzb = make_fastdiv((uint32_t)cc.nzb);
+    const uint32_t nbl = (uint32_t)cc.nzb * g.n[3] * g.n[4];
+    const uint32_t nbh = ((nbl + 7u) / 8u) * 8u;
+    const size_t smem = 2 * ZC * (size_t)cc.nxyp * sizeof(T);
+    const dim3 blk((uint32_t)cc.nxyp);
+    if (homog) {
+        if (is_full<true>(g))
+            hipLaunchKernelGGL((k_step_colp<T, EXACT, true, ZC, true>), dim3(nbh), blk, smem, stream, g, p, cc, nbl);
+        else
+            hipLaunchKernelGGL((k_step_colp<T, EXACT, true, ZC, false>), dim3(nbh), blk, smem, stream, g, p, cc, nbl);
+    } else {
+        if (is_full<false>(g))
+            hipLaunchKernelGGL((k_step_colp<T, EXACT, false, ZC, true>), dim3(nbh), blk, smem, stream, g, p, cc, nbl);
+        else
+            hipLaunchKernelGGL((k_step_colp<T, EXACT, false, ZC, false>), dim3(nbh), blk, smem, stream, g, p, cc, nbl);
     }
     return hipGetLastError();
 }
@@ -520,15 +557,21 @@ hipError_t launch_step_impl(const Geo &g, const StepPtrs<T> &p, bool homog, cons
     const bool col_fits = (uint64_t)g.ncells * 2u * sizeof(T) < (1ull << 32);
     if (lc.kernel >= 9 && lc.kernel <= 11 && col_fits) {
         const int zc = lc.kernel == 9 ? 1 : (lc.kernel == 10 ? 2 : 4);
-        if (zc == 4 && g.n[2] % 4 == 0) return launch_col<T, EXACT, 4, true>(g, p, homog, lc.stream);
-        if (zc >= 2 && g.n[2] % 2 == 0) return launch_col<T, EXACT, 2, true>(g, p, homog, lc.stream);
-        return launch_col<T, EXACT, 1, true>(g, p, homog, lc.stream);
+        if (zc == 4 && g.n[2] % 4 == 0) return launch_col<T, EXACT, 4, 1>(g, p, homog, lc.stream);
+        if (zc >= 2 && g.n[2] % 2 == 0) return launch_col<T, EXACT, 2, 1>(g, p, homog, lc.stream);
+        return launch_col<T, EXACT, 1, 1>(g, p, homog, lc.stream);
+    }
+    if (lc.kernel >= 12 && lc.kernel <= 14 && col_fits && colp_fits<T>(g, lc.kernel == 12 ? 1 : (lc.kernel == 13 ? 2 : 4))) {
+        const int zc = lc.kernel == 12 ? 1 : (lc.kernel == 13 ? 2 : 4);
+        if (zc == 4) return launch_colp<T, EXACT, 4>(g, p, homog, lc.stream);
+        if (zc == 2) return launch_colp<T, EXACT, 2>(g, p, homog, lc.stream);
+        return launch_colp<T, EXACT, 1>(g, p, homog, lc.stream);
     }
     if (lc.kernel == 0 && !homog && col_fits) {
         // general path: the y neighbours from LDS (measured 15% faster at
         // 24^3 x 16^3 in binary64; the homogeneous path is faster without)
-        if (g.n[2] % 2 == 0) return launch_col<T, EXACT, 2, true>(g, p, homog, lc.stream);
-        return launch_col<T, EXACT, 1, true>(g, p, homog, lc.stream);
+        if (g.n[2] % 2 == 0) return launch_col<T, EXACT, 2, 1>(g, p, homog, lc.stream);
+        return launch_col<T, EXACT, 1, 1>(g, p, homog, lc.stream);
     }
     if (lc.kernel != 1 && col_fits) {
         const int zc = col_zc(g, lc.kernel);
@@ -626,6 +669,9 @@ hipError_t launch_vmax(const Geo &g, const double *const vel[kAxes], unsigned lo
 hipError_t launch_cfl_max(const Geo &g, const double *const vel[kAxes], const AxisCoef &ax,
                           unsigned long long *bits, hipStream_t s);
 hipError_t launch_convert_d2f(const double *in, float *out, size_t n, hipStream_t s);
+template <typename T>
+hipError_t launch_bottom_forcing(const Geo &g, int kind, T *E, const double *F, const T *zdot, double dt, double dz1,
+                                 hipStream_t s);
 template <typename T>
 hipError_t launch_init_separable(const Geo &g, const double *f, double peak, double noise, uint64_t key, T *E,
                                  hipStream_t s);

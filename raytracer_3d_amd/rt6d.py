@@ -40,7 +40,8 @@ EXPORTS = ("rt6d_version", "rt6d_last_error", "rt6d_device_count", "rt6d_create"
            "rt6d_set_option", "rt6d_query", "rt6d_synchronize", "rt6d_copy_bandwidth",
            "rt6d_ensemble_create", "rt6d_ensemble_destroy", "rt6d_ensemble_add", "rt6d_ensemble_remove",
            "rt6d_ensemble_step", "rt6d_ensemble_set_schedule", "rt6d_ensemble_tune", "rt6d_ensemble_info",
-           "rt6d_init_separable", "rt6d_branch_flips", "rt6d_ensemble_step_async", "rt6d_ensemble_wait")
+           "rt6d_init_separable", "rt6d_branch_flips", "rt6d_ensemble_step_async", "rt6d_ensemble_wait",
+           "rt6d_set_forcing")
 
 
 class Diag(C.Structure):
@@ -109,6 +110,7 @@ def load_library(path: str | None = None):
         "rt6d_branch_flips": ([vp, vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)], C.c_int),
         "rt6d_ensemble_step_async": ([vp, C.c_int], C.c_int),
         "rt6d_ensemble_wait": ([vp, dp], C.c_int),
+        "rt6d_set_forcing": ([vp, C.c_int, dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -222,6 +224,16 @@ class Model:
             raise ValueError("factors must hold n0+...+n5 values")
         _check(self.lib, self.lib.rt6d_init_separable(self._h, _dptr(f), peak, noise, seed, member),
                "rt6d_init_separable")
+
+    def set_forcing(self, kind: int, field=None):
+        """Bottom forcing before every step: 0 none, 1 flux injection, 2 relaxation (rt6d_set_forcing)."""
+        if kind == 0:
+            _check(self.lib, self.lib.rt6d_set_forcing(self._h, 0, None), "rt6d_set_forcing")
+            return
+        f = np.ascontiguousarray(field, dtype=np.float64).ravel()
+        if f.size != self.cells // self.n[2]:
+            raise ValueError("forcing field must hold nx*ny*nk*nl*nm values")
+        _check(self.lib, self.lib.rt6d_set_forcing(self._h, kind, _dptr(f)), "rt6d_set_forcing")
 
     @classmethod
     def from_case(cls, case: "_cases.Case", precision=FP64, exact=False, path=0, block=None,
