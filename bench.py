@@ -135,6 +135,8 @@ def _step_kernel(precision, homog, nz):
     compiled for all advected axes active, y neighbours from LDS on the
     general path)."""
     zc = 2 if nz % 2 == 0 else 1
+    if homog and precision == "double":
+        return f"k_step_colx2<double, false, {zc}, true>"
     h = "true" if homog else "false"
     ym = 0 if homog else 1
     return f"k_step_col<{precision}, false, {h}, {zc}, true, {ym}>"

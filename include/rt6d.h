@@ -64,8 +64,10 @@ extern "C" {
                                 (6D coefficient fields streamed from HBM) */
 #define RT6D_OPT_BLOCK 3     /* threads per block of the per-cell step kernel (64..1024) */
 #define RT6D_OPT_KERNEL 4    /* step kernel: 0 auto (default: the column kernel with two z
-                                cells per thread when nz is even, with the y neighbours
-                                staged in LDS on the general path); 1 per-cell kernel;
+                                cells per thread when nz is even, with two x cells per
+                                thread on the binary64 homogeneous path when nx is even,
+                                with the y neighbours staged in LDS on the general path);
+                                1 per-cell kernel;
                                 2/3 LDS marching kernel, 1/2 z cells per thread (nx*ny <= 1024);
                                 4/5/6 column kernel, 1/2/4 z cells per thread;
                                 7/8 plane kernel with x/y face-flux exchange in LDS,
@@ -73,7 +75,10 @@ extern "C" {
                                 9/10/11 column kernel with the y neighbours staged in
                                 LDS, 1/2/4 z cells per thread;
                                 12/13/14 the same with one block per (x,y) plane set, every
-                                y neighbour from LDS (nx*ny <= 1024) */
+                                y neighbour from LDS (nx*ny <= 1024);
+                                15/16/17 column kernel with two x cells per thread (paired
+                                accesses), 1/2/4 z cells per thread: homogeneous path with
+                                nx even, else the default */
 
 /* queries for rt6d_query */
 #define RT6D_Q_HOMOGENEOUS 1 /* 1 if the horizontally homogeneous path is active */

@@ -1,0 +1,276 @@
+// rt6d_colx2.h -- the column step kernel with two x cells per thread
+// (included by rt6d_kernels.h after rt6d_column.h, inside namespace rt6d).
+//
+// The column kernel's march (register window along m, carried m-face flux,
+// ZC z cells per thread sharing their inner z faces) with each thread owning
+// the pair of x cells (x, x+1), x odd, so that every access moves a pair: the
+// own pair, the pairs (x-2, x-1) and (x+2, x+3) for the x stencil, the pairs
+// of the four neighbouring rows for y and of the four outer z planes.  The
+// pair shares its inner x face, so three x faces serve two cells.  Halving
+// the vector-memory instructions per cell is the point: the column kernel is
+// bounded by the vector-memory pipeline, not by HBM or the VALU.
+// Horizontally homogeneous path only, nx even (the pairs then stay aligned
+// with the rows); the face arithmetic and the order in which each cell sums
+// its axes are those of the column kernel, so results are bit for bit the
+// same.
+#pragma once
+
+template <typename T>
+struct Pair {
+    T a, b;
+};
+
+template <typename T>
+__device__ __forceinline__ Pair<T> bld2(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
+template <>
+__device__ __forceinline__ Pair<double> bld2<double>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    using v4i = int __attribute__((ext_vector_type(4)));
+    const v4i v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0);
+    using v2d = double __attribute__((ext_vector_type(2)));
+    const v2d d = __builtin_bit_cast(v2d, v);
+    return Pair<double>{d.x, d.y};
+}
+template <>
+__device__ __forceinline__ Pair<float> bld2<float>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    using v2i = int __attribute__((ext_vector_type(2)));
+    const v2i v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0);
+    using v2f = float __attribute__((ext_vector_type(2)));
+    const v2f f = __builtin_bit_cast(v2f, v);
+    return Pair<float>{f.x, f.y};
+}
+template <typename T>
+__device__ __forceinline__ void bst2(Pair<T> v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
+template <>
+__device__ __forceinline__ void bst2<double>(Pair<double> v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    using v4i = int __attribute__((ext_vector_type(4)));
+    using v2d = double __attribute__((ext_vector_type(2)));
+    v2d d;
+    d.x = v.a;
+    d.y = v.b;
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4i, d), r, (int)voff, (int)soff, 0);
+}
+template <>
+__device__ __forceinline__ void bst2<float>(Pair<float> v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    using v2i = int __attribute__((ext_vector_type(2)));
+    using v2f = float __attribute__((ext_vector_type(2)));
+    v2f f;
+    f.x = v.a;
+    f.y = v.b;
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2i, f), r, (int)voff, (int)soff, 0);
+}
+
+// Column configuration of the pair kernel: the padded plane counts pairs.
+struct ColX2Cfg {
+    FastDiv dxyp;  // division by the padded (pair, y) plane size
+    FastDiv dzb;   // division by the number of z blocks
+    FastDiv dxh;   // division by nx/2
+    int nxyp;      // (nx/2)*ny rounded up to a multiple of 64
+    int nzb;       // nz / ZC
+};
+
+template <typename T, bool EXACT, int ZC, bool FULL>
+__device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, const ColX2Cfg &cc, uint32_t b)
+{
+    const uint32_t t = b * blockDim.x + threadIdx.x;
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2], nk = g.n[3], nl = g.n[4], nm = g.n[5];
+    const int nxy = nx * ny, nxh = nx >> 1;
+    const uint32_t pl = __builtin_amdgcn_readfirstlane(fdiv(t, cc.dxyp));
+    const int xyp = (int)(t - pl * (uint32_t)cc.nxyp);
+    const uint32_t kl = fdiv(pl, cc.dzb);
+    const int zb = (int)(pl - kl * (uint32_t)cc.nzb);
+    const int bl = (int)fdiv(kl, g.div[3]);
+    const int bk = (int)kl - bl * nk;
+    if (bl >= nl) return;          // whole wavefronts beyond the grid
+    if (xyp >= nxh * ny) return;   // padding lanes (no barriers below)
+    const int y0 = (int)fdiv((uint32_t)xyp, cc.dxh);
+    const int x0 = 2 * (xyp - y0 * nxh);   // 0-based first cell of the pair
+    const int x = x0 + 1, y = y0 + 1;      // 1-based: the pair is (x, x+1)
+    const int z0 = zb * ZC + 1, k = bk + 1, l = bl + 1;
+    constexpr uint32_t S = sizeof(T);
+
+    const uint32_t nzkl = (uint32_t)nz * nk * nl;
+    const uint32_t Qkl = (uint32_t)bk + (uint32_t)nk * (uint32_t)bl;
+    const uint32_t P0 = (uint32_t)(z0 - 1) + (uint32_t)nz * Qkl;
+    const uint64_t cellsB = (uint64_t)g.ncells * S;
+    const __amdgpu_buffer_rsrc_t rE = mk_rsrc(p.Ein, cellsB);
+    const __amdgpu_buffer_rsrc_t rO = mk_rsrc(p.Eout, cellsB);
+    const uint32_t planeB = (uint32_t)nxy * S;
+    const uint32_t mB = (uint32_t)g.inner[5] * S;
+    const uint32_t base0 = P0 * planeB;
+    const bool ax = FULL || (g.act & 1), ay = FULL || (g.act & 2), az = FULL || (g.act & 4);
+    const bool am = FULL || (g.act & 32);
+    const double dtv = p.dt;
+
+    const int xy0 = x0 + nx * y0;
+    const uint32_t oc = (uint32_t)xy0 * S;
+    // x stencil pairs: (x-2, x-1) and (x+2, x+3); at the row ends the
+    // zero-gradient ghosts repeat the pair's own edge cell
+    const bool xfirst = x0 == 0, xlast = x0 + 2 >= nx;
+    const uint32_t oxl = xfirst ? oc : oc - 2 * S, oxr = xlast ? oc : oc + 2 * S;
+    uint32_t oy[4];
+    {
+        const int ys[4] = {y - 2 < 1 ? 1 : y - 2, y - 1 < 1 ? 1 : y - 1, y + 1 > ny ? ny : y + 1,
+                           y + 2 > ny ? ny : y + 2};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) oy[q] = (uint32_t)(x0 + nx * (ys[q] - 1)) * S;
+    }
+    // x faces x-1, x, x+1 and cells x, x+1; y faces y-1, y of row y
+    double cx0 = 0, cx1 = 0, cx2 = 0, dx0 = 1, dx1 = 1, dx2 = 1, rxa = 0, rxb = 0;
+    if (ax) {
+        cx0 = p.ax.cfl[0][x - 1]; cx1 = p.ax.cfl[0][x]; cx2 = p.ax.cfl[0][x + 1];
+        dx0 = p.ax.dx[0][x - 1]; dx1 = p.ax.dx[0][x]; dx2 = p.ax.dx[0][x + 1];
+        rxa = p.ax.rdx[0][x]; rxb = p.ax.rdx[0][x + 1];
+    }
+    double cyl = 0, cyr = 0, dyl = 1, dyr = 1, ryy = 0;
+    if (ay) { cyl = p.ax.cfl[1][y - 1]; cyr = p.ax.cfl[1][y]; dyl = p.ax.dx[1][y - 1]; dyr = p.ax.dx[1][y]; ryy = p.ax.rdx[1][y]; }
+
+    int zsh[4];
+    bool zmir[4];
+    {
+        const int zz0[4] = {z0 - 2, z0 - 1, z0 + ZC, z0 + ZC + 1};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            int zz = zz0[q];
+            bool mir = false;
+            if (zz < 1) { if (g.down) { zz = 1 - zz; mir = true; } else zz = 1; }
+            else if (zz > nz) { if (g.up) { zz = 2 * nz + 1 - zz; mir = true; } else zz = nz; }
+            zsh[q] = zz - z0;
+            zmir[q] = mir;
+        }
+    }
+    auto poff = [&](int j, int mm) -> uint32_t { return base0 + (uint32_t)j * planeB + (uint32_t)(mm - 1) * mB; };
+    auto ldm = [&](int j, int mm) -> Pair<T> {
+        mm = mm < 1 ? 1 : (mm > nm ? nm : mm);
+        return bld2<T>(rE, oc, poff(j, mm));
+    };
+    auto ldz = [&](int q, int mm) -> Pair<T> {
+        const int mu = zmir[q] ? nm + 1 - mm : mm;
+        return bld2<T>(rE, oc, base0 + (uint32_t)(zsh[q] * (int)planeB) + (uint32_t)(mu - 1) * mB);
+    };
+    Pair<T> wm1[ZC], w0[ZC], wp1[ZC], wp2[ZC], FLm[ZC];
+#pragma unroll
+    for (int j = 0; j < ZC; ++j) {
+        w0[j] = ldm(j, 1);
+        wp1[j] = ldm(j, 2);
+        wp2[j] = ldm(j, 3);
+        wm1[j] = w0[j];
+        FLm[j] = Pair<T>{T(0), T(0)};
+    }
+    Pair<T> zq[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) zq[q] = az ? ldz(q, 1) : Pair<T>{T(0), T(0)};
+    if (am) {
+#pragma unroll
+        for (int j = 0; j < ZC; ++j) {
+            const T vm0 = sld(p.hom.vm, P0 + j);
+            FLm[j] = Pair<T>{vm0 * w0[j].a, vm0 * w0[j].b};   // m face 0: v*E(1)
+        }
+    }
+
+    for (int m = 1; m <= nm; ++m) {
+        const uint32_t q3 = Qkl + (uint32_t)(m - 1) * (uint32_t)nk * nl;
+        const uint32_t q20 = P0 + (uint32_t)(m - 1) * nzkl;
+        Pair<T> wnx[ZC], znx[4];
+#pragma unroll
+        for (int j = 0; j < ZC; ++j) wnx[j] = ldm(j, m + 3);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) znx[q] = (az && m < nm) ? ldz(q, m + 1) : Pair<T>{T(0), T(0)};
+
+        T FzA[ZC + 1], FzB[ZC + 1];
+        if (az) {
+            T za[ZC + 4], zbv[ZC + 4];
+            za[0] = zq[0].a; za[1] = zq[1].a; zbv[0] = zq[0].b; zbv[1] = zq[1].b;
+#pragma unroll
+            for (int j = 0; j < ZC; ++j) { za[2 + j] = w0[j].a; zbv[2 + j] = w0[j].b; }
+            za[ZC + 2] = zq[2].a; za[ZC + 3] = zq[3].a; zbv[ZC + 2] = zq[2].b; zbv[ZC + 3] = zq[3].b;
+#pragma unroll
+            for (int i = 0; i <= ZC; ++i) {
+                const int fz = z0 - 1 + i;
+                const T v = sld(p.hom.vz, (uint32_t)fz + (uint32_t)(nz + 1) * q3);
+                const double dzf = sld(p.ax.dx[2], fz), czf = sld(p.ax.cfl[2], fz);
+                FzA[i] = face_flux<T, EXACT>(v, dtv, dzf, czf, za[i], za[i + 1], za[i + 2], za[i + 3]);
+                FzB[i] = face_flux<T, EXACT>(v, dtv, dzf, czf, zbv[i], zbv[i + 1], zbv[i + 2], zbv[i + 3]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < ZC; ++j) {
+            const uint32_t Pm = P0 + (uint32_t)j + (uint32_t)(m - 1) * nzkl;
+            const uint32_t pb = Pm * planeB;
+            const uint32_t q2 = q20 + (uint32_t)j;
+            const Pair<T> e = w0[j];
+            T dEa = T(0), dEb = T(0);
+            if (ax) {
+                Pair<T> L = bld2<T>(rE, oxl, pb), R = bld2<T>(rE, oxr, pb);
+                if (xfirst) L = Pair<T>{e.a, e.a};
+                if (xlast) R = Pair<T>{e.b, e.b};
+                const T v0 = sld(p.hom.vx3, q2), vi = sld(p.hom.vx3, p.hom.n4 + q2), vb = sld(p.hom.vx3, 2 * p.hom.n4 + q2);
+                const T vL = x == 1 ? v0 : vi, vR = x + 1 == nx ? vb : vi;
+                const T F0 = face_flux<T, EXACT>(vL, dtv, dx0, cx0, L.a, L.b, e.a, e.b);
+                const T F1 = face_flux<T, EXACT>(vi, dtv, dx1, cx1, L.b, e.a, e.b, R.a);
+                const T F2 = face_flux<T, EXACT>(vR, dtv, dx2, cx2, e.a, e.b, R.a, R.b);
+                dEa = div_term<T, EXACT>(dEa, F1, F0, dx1, rxa);
+                dEb = div_term<T, EXACT>(dEb, F2, F1, dx2, rxb);
+            }
+            if (ay) {
+                const Pair<T> a0 = bld2<T>(rE, oy[0], pb), a1 = bld2<T>(rE, oy[1], pb);
+                const Pair<T> a3 = bld2<T>(rE, oy[2], pb), a4 = bld2<T>(rE, oy[3], pb);
+                const T v0 = sld(p.hom.vy3, q2), vi = sld(p.hom.vy3, p.hom.n4 + q2), vb = sld(p.hom.vy3, 2 * p.hom.n4 + q2);
+                const T vL = y == 1 ? v0 : vi, vR = y == ny ? vb : vi;
+                const T FLa = face_flux<T, EXACT>(vL, dtv, dyl, cyl, a0.a, a1.a, e.a, a3.a);
+                const T FRa = face_flux<T, EXACT>(vR, dtv, dyr, cyr, a1.a, e.a, a3.a, a4.a);
+                const T FLb = face_flux<T, EXACT>(vL, dtv, dyl, cyl, a0.b, a1.b, e.b, a3.b);
+                const T FRb = face_flux<T, EXACT>(vR, dtv, dyr, cyr, a1.b, e.b, a3.b, a4.b);
+                dEa = div_term<T, EXACT>(dEa, FRa, FLa, dyr, ryy);
+                dEb = div_term<T, EXACT>(dEb, FRb, FLb, dyr, ryy);
+            }
+            if (az) {
+                const int zc = z0 + j;
+                const double dzc = sld(p.ax.dx[2], zc), rzc = sld(p.ax.rdx[2], zc);
+                dEa = div_term<T, EXACT>(dEa, FzA[j + 1], FzA[j], dzc, rzc);
+                dEb = div_term<T, EXACT>(dEb, FzB[j + 1], FzB[j], dzc, rzc);
+            }
+            if (am) {
+                const T vR = sld(p.hom.vm, q2 + p.hom.inner4m);
+                const double dm = sld(p.ax.dx[5], m), cm = sld(p.ax.cfl[5], m), rm = sld(p.ax.rdx[5], m);
+                const T FRa = face_flux<T, EXACT>(vR, dtv, dm, cm, wm1[j].a, e.a, wp1[j].a, wp2[j].a);
+                const T FRb = face_flux<T, EXACT>(vR, dtv, dm, cm, wm1[j].b, e.b, wp1[j].b, wp2[j].b);
+                dEa = div_term<T, EXACT>(dEa, FRa, FLm[j].a, dm, rm);
+                dEb = div_term<T, EXACT>(dEb, FRb, FLm[j].b, dm, rm);
+                FLm[j] = Pair<T>{FRa, FRb};
+            }
+            const T lam = sld(p.hom.lam, q2);
+            dEa = dEa - e.a * lam;
+            dEb = dEb - e.b * lam;
+            Pair<T> en;
+            if constexpr (EXACT) {
+                en.a = e.a + p.dt * dEa;
+                en.b = e.b + p.dt * dEb;
+            } else {
+                en.a = e.a + (T)p.dt * dEa;
+                en.b = e.b + (T)p.dt * dEb;
+            }
+            bst2<T>(en, rO, oc, pb);
+        }
+#pragma unroll
+        for (int j = 0; j < ZC; ++j) {
+            wm1[j] = w0[j];
+            w0[j] = wp1[j];
+            wp1[j] = wp2[j];
+            wp2[j] = wnx[j];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) zq[q] = znx[q];
+    }
+}
+
+template <typename T, bool EXACT, int ZC, bool FULL>
+__global__ void __launch_bounds__(256) k_step_colx2(const Geo g, const StepPtrs<T> p, const ColX2Cfg cc, uint32_t nbl)
+{
+    const uint32_t b = xcd_logical_block(blockIdx.x, nbl);
+    if (b >= nbl) return;
+    colx2_body<T, EXACT, ZC, FULL>(g, p, cc, b);
+}

@@ -23,7 +23,9 @@ struct EnsMember {
     Geo g;
     StepPtrs<T> p[2];   // p[0]: E[a] -> E[b]; p[1]: E[b] -> E[a]
     ColCfg cc;
-    uint32_t variant;   // bit 0: homogeneous tables; bit 1: two z cells per thread; bit 2: all axes active
+    ColX2Cfg cx;        // pair-kernel configuration (variants with bit 3)
+    uint32_t variant;   // bit 0: homogeneous tables; bit 1: two z cells per thread; bit 2: all axes
+                        // active; bit 3: two x cells per thread (binary64, homogeneous, nx even)
     uint32_t pad;
 };
 
@@ -31,7 +33,7 @@ struct EnsTask {
     uint32_t member, block;
 };
 
-constexpr int kEnsVariants = 8;
+constexpr int kEnsVariants = 16;
 constexpr int kEnsCounters = kEnsVariants * 8;   // one work counter per variant and XCD
 
 static __global__ void k_ens_reset(uint32_t *ctr)
@@ -39,7 +41,14 @@ static __global__ void k_ens_reset(uint32_t *ctr)
     if (threadIdx.x < kEnsCounters) ctr[threadIdx.x] = 0;
 }
 
-template <typename T, bool HOMOG, int ZC, bool FULL, bool PERSIST>
+template <typename T, bool HOMOG, int ZC, bool FULL, bool X2>
+__device__ __forceinline__ void ens_task(const EnsMember<T> &M, uint32_t parity, uint32_t block)
+{
+    if constexpr (X2) colx2_body<T, false, ZC, FULL>(M.g, M.p[parity & 1u], M.cx, block);
+    else col_body<T, false, HOMOG, ZC, FULL>(M.g, M.p[parity & 1u], M.cc, block);
+}
+
+template <typename T, bool HOMOG, int ZC, bool FULL, bool X2, bool PERSIST>
 __global__ void __launch_bounds__(256) k_ens_step(const EnsMember<T> *__restrict__ mem,
                                                   const EnsTask *__restrict__ tasks, uint32_t ntasks,
                                                   uint32_t parity, uint32_t *ctr)
@@ -48,8 +57,7 @@ __global__ void __launch_bounds__(256) k_ens_step(const EnsMember<T> *__restrict
         const uint32_t t = xcd_logical_block(blockIdx.x, ntasks);
         if (t >= ntasks) return;   // uniform over the block
         const EnsTask tk = tasks[t];
-        const EnsMember<T> &M = mem[tk.member];
-        col_body<T, false, HOMOG, ZC, FULL>(M.g, M.p[parity & 1u], M.cc, tk.block);
+        ens_task<T, HOMOG, ZC, FULL, X2>(mem[tk.member], parity, tk.block);
     } else {
         __shared__ uint32_t s_task;
         const uint32_t xcd = blockIdx.x & 7u;
@@ -63,8 +71,7 @@ __global__ void __launch_bounds__(256) k_ens_step(const EnsMember<T> *__restrict
             __syncthreads();   // every thread has read s_task before thread 0 writes it again
             if (t >= t1) break;
             const EnsTask tk = tasks[t];
-            const EnsMember<T> &M = mem[tk.member];
-            col_body<T, false, HOMOG, ZC, FULL>(M.g, M.p[parity & 1u], M.cc, tk.block);
+            ens_task<T, HOMOG, ZC, FULL, X2>(mem[tk.member], parity, tk.block);
         }
     }
 }

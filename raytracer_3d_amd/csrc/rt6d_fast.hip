@@ -47,17 +47,19 @@ hipError_t launch_diag_fast(const Geo &g, const StepPtrs<T> &p, bool homog, cons
     return hipGetLastError();
 }
 
-template <typename T, bool HOMOG, int ZC, bool FULL>
+template <typename T, bool HOMOG, int ZC, bool FULL, bool X2 = false>
 static void ens_launch_variant(const EnsMember<T> *mem, const EnsTask *tasks, uint32_t n, uint32_t parity,
                                uint32_t persist_blocks, uint32_t *ctr, hipStream_t s)
 {
     if (persist_blocks) {
         uint32_t nb = persist_blocks < n ? persist_blocks : n;
         nb = ((nb + 7u) / 8u) * 8u;
-        hipLaunchKernelGGL((k_ens_step<T, HOMOG, ZC, FULL, true>), dim3(nb), dim3(256), 0, s, mem, tasks, n, parity, ctr);
+        hipLaunchKernelGGL((k_ens_step<T, HOMOG, ZC, FULL, X2, true>), dim3(nb), dim3(256), 0, s, mem, tasks, n, parity,
+                           ctr);
     } else {
         const uint32_t nb = ((n + 7u) / 8u) * 8u;
-        hipLaunchKernelGGL((k_ens_step<T, HOMOG, ZC, FULL, false>), dim3(nb), dim3(256), 0, s, mem, tasks, n, parity, ctr);
+        hipLaunchKernelGGL((k_ens_step<T, HOMOG, ZC, FULL, X2, false>), dim3(nb), dim3(256), 0, s, mem, tasks, n,
+                           parity, ctr);
     }
 }
 
@@ -79,7 +81,13 @@ hipError_t launch_ensemble(const EnsMember<T> *mem, const EnsTask *tasks, const 
         case 4: ens_launch_variant<T, false, 1, true>(mem, tv, cnt[v], parity, persist_blocks, cv, s); break;
         case 5: ens_launch_variant<T, true, 1, true>(mem, tv, cnt[v], parity, persist_blocks, cv, s); break;
         case 6: ens_launch_variant<T, false, 2, true>(mem, tv, cnt[v], parity, persist_blocks, cv, s); break;
-        default: ens_launch_variant<T, true, 2, true>(mem, tv, cnt[v], parity, persist_blocks, cv, s); break;
+        case 7: ens_launch_variant<T, true, 2, true>(mem, tv, cnt[v], parity, persist_blocks, cv, s); break;
+        // pair kernel: binary64 homogeneous members only (bit 0 set)
+        case 9: ens_launch_variant<T, true, 1, false, true>(mem, tv, cnt[v], parity, persist_blocks, cv, s); break;
+        case 11: ens_launch_variant<T, true, 2, false, true>(mem, tv, cnt[v], parity, persist_blocks, cv, s); break;
+        case 13: ens_launch_variant<T, true, 1, true, true>(mem, tv, cnt[v], parity, persist_blocks, cv, s); break;
+        case 15: ens_launch_variant<T, true, 2, true, true>(mem, tv, cnt[v], parity, persist_blocks, cv, s); break;
+        default: break;   // not produced by the host
         }
     }
     return hipGetLastError();

@@ -167,6 +167,7 @@ __device__ __forceinline__ T div_term(T dE, T FR, T FL, double dxc, double rdxc)
 #include "rt6d_column.h"
 #include "rt6d_march.h"
 #include "rt6d_plane.h"
+#include "rt6d_colx2.h"
 #include "rt6d_ensemble.h"
 
 // Tendency fields of one step (the by-products integrate leaves behind:
@@ -529,6 +530,32 @@ hipError_t launch_colp(const Geo &g, const StepPtrs<T> &p, bool homog, hipStream
     return hipGetLastError();
 }
 
+// Pair kernel (two x cells per thread): homogeneous path, nx even.
+inline bool colx2_fits(const Geo &g, bool homog, int zc)
+{
+    return homog && g.n[0] >= 2 && g.n[0] % 2 == 0 && g.n[2] % zc == 0;
+}
+
+template <typename T, bool EXACT, int ZC>
+hipError_t launch_colx2(const Geo &g, const StepPtrs<T> &p, hipStream_t stream)
+{
+    const uint32_t npairs = (uint32_t)(g.n[0] / 2) * g.n[1];
+    ColX2Cfg cc;
+    cc.nxyp = (int)(((npairs + 63u) / 64u) * 64u);
+    cc.nzb = g.n[2] / ZC;
+    cc.dxyp = make_fastdiv((uint32_t)cc.nxyp);
+    cc.dzb = make_fastdiv((uint32_t)cc.nzb);
+    cc.dxh = make_fastdiv((uint32_t)(g.n[0] / 2));
+    const uint64_t nthreads = (uint64_t)cc.nxyp * cc.nzb * g.n[3] * g.n[4];
+    const uint32_t nbl = (uint32_t)((nthreads + 255) / 256);
+    const uint32_t nbh = ((nbl + 7u) / 8u) * 8u;
+    if (is_full<true>(g))
+        hipLaunchKernelGGL((k_step_colx2<T, EXACT, ZC, true>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
+    else
+        hipLaunchKernelGGL((k_step_colx2<T, EXACT, ZC, false>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
+    return hipGetLastError();
+}
+
 // z cells per thread of the column kernel: two when nz is even (measured
 // fastest at 24^3 x 16^3 in binary64 and binary32), else one
 inline int col_zc(const Geo &g, int kernel)
@@ -561,11 +588,25 @@ hipError_t launch_step_impl(const Geo &g, const StepPtrs<T> &p, bool homog, cons
         if (zc >= 2 && g.n[2] % 2 == 0) return launch_col<T, EXACT, 2, 1>(g, p, homog, lc.stream);
         return launch_col<T, EXACT, 1, 1>(g, p, homog, lc.stream);
     }
+    if (lc.kernel >= 15 && lc.kernel <= 17 && col_fits) {
+        const int zc = lc.kernel == 15 ? 1 : (lc.kernel == 16 ? 2 : 4);
+        if (colx2_fits(g, homog, zc)) {
+            if (zc == 4) return launch_colx2<T, EXACT, 4>(g, p, lc.stream);
+            if (zc == 2) return launch_colx2<T, EXACT, 2>(g, p, lc.stream);
+            return launch_colx2<T, EXACT, 1>(g, p, lc.stream);
+        }
+    }
     if (lc.kernel >= 12 && lc.kernel <= 14 && col_fits && colp_fits<T>(g, lc.kernel == 12 ? 1 : (lc.kernel == 13 ? 2 : 4))) {
         const int zc = lc.kernel == 12 ? 1 : (lc.kernel == 13 ? 2 : 4);
         if (zc == 4) return launch_colp<T, EXACT, 4>(g, p, homog, lc.stream);
         if (zc == 2) return launch_colp<T, EXACT, 2>(g, p, homog, lc.stream);
         return launch_colp<T, EXACT, 1>(g, p, homog, lc.stream);
+    }
+    if (lc.kernel == 0 && homog && col_fits && sizeof(T) == 8 && colx2_fits(g, homog, 1)) {
+        // binary64 homogeneous path: paired accesses (measured 10% faster at
+        // 24^3 x 16^3; no gain in binary32)
+        if (g.n[2] % 2 == 0) return launch_colx2<T, EXACT, 2>(g, p, lc.stream);
+        return launch_colx2<T, EXACT, 1>(g, p, lc.stream);
     }
     if (lc.kernel == 0 && !homog && col_fits) {
         // general path: the y neighbours from LDS (measured 15% faster at

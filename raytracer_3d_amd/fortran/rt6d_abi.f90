@@ -122,6 +122,61 @@ module rt6d_abi
       type(c_ptr), value :: ctx
       integer(c_int) :: rt6d_synchronize
     end function
+    ! set_forcing on the device (kind 0 none, 1 bottom flux, 2 bottom relaxation)
+    function rt6d_set_forcing(ctx, kind, field) bind(C, name='rt6d_set_forcing')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: kind
+      real(c_double), intent(in) :: field(*)
+      integer(c_int) :: rt6d_set_forcing
+    end function
+    function rt6d_init_separable(ctx, factors, peak, noise, seed, member) bind(C, name='rt6d_init_separable')
+      import :: c_int, c_ptr, c_double, c_long_long
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: factors(*)
+      real(c_double), value :: peak, noise
+      integer(c_long_long), value :: seed, member
+      integer(c_int) :: rt6d_init_separable
+    end function
+    ! ensembles of independent integrations
+    function rt6d_ensemble_create(prec, ens) bind(C, name='rt6d_ensemble_create')
+      import :: c_int, c_ptr
+      integer(c_int), value :: prec
+      type(c_ptr), intent(out) :: ens
+      integer(c_int) :: rt6d_ensemble_create
+    end function
+    function rt6d_ensemble_destroy(ens) bind(C, name='rt6d_ensemble_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ens
+      integer(c_int) :: rt6d_ensemble_destroy
+    end function
+    function rt6d_ensemble_add(ens, member, dt, slot) bind(C, name='rt6d_ensemble_add')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ens, member
+      real(c_double), value :: dt
+      integer(c_int), intent(out) :: slot
+      integer(c_int) :: rt6d_ensemble_add
+    end function
+    function rt6d_ensemble_remove(ens, slot) bind(C, name='rt6d_ensemble_remove')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ens
+      integer(c_int), value :: slot
+      integer(c_int) :: rt6d_ensemble_remove
+    end function
+    function rt6d_ensemble_step(ens, nsteps, ms) bind(C, name='rt6d_ensemble_step')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ens
+      integer(c_int), value :: nsteps
+      real(c_double), intent(out) :: ms
+      integer(c_int) :: rt6d_ensemble_step
+    end function
+    function rt6d_ensemble_tune(ens, trial_steps, blocks_per_cu, order) bind(C, name='rt6d_ensemble_tune')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ens
+      integer(c_int), value :: trial_steps
+      integer(c_int), intent(out) :: blocks_per_cu, order
+      integer(c_int) :: rt6d_ensemble_tune
+    end function
   end interface
 
 contains

@@ -44,18 +44,18 @@ def test_calc_parameter_bitwise(entry, gpu_available):
         assert m.homogeneous == (entry["stem"] in HOMOG_EXPECTED)
 
 
-@pytest.mark.parametrize("kernel", [0, 1, 2, 3, 4, 5, 7, 8, 9, 10, 11, 12, 13, 14],
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3, 4, 5, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17],
                          ids=["auto", "percell", "march1", "march2", "col1", "col2", "plane1", "plane2",
-                              "coly1", "coly2", "coly4", "colp1", "colp2", "colp4"])
+                              "coly1", "coly2", "coly4", "colp1", "colp2", "colp4", "colx1", "colx2", "colx4"])
 @pytest.mark.parametrize("exact", [True, False], ids=["exact", "fast"])
 @pytest.mark.parametrize("path", [0, 1], ids=["auto", "general"])
 @pytest.mark.parametrize("entry", ENTRIES, ids=IDS)
 def test_integrate_matches_reference(entry, path, exact, kernel, gpu_available):
     case = golden_case(entry)
     g = golden_data(entry)
-    if kernel in (3, 5, 8, 10, 13) and case.n[2] % 2:
+    if kernel in (3, 5, 8, 10, 13, 16) and case.n[2] % 2:
         pytest.skip("two z cells per thread need an even nz")
-    if kernel in (11, 14) and case.n[2] % 4:
+    if kernel in (11, 14, 17) and case.n[2] % 4:
         pytest.skip("four z cells per thread need nz divisible by 4")
     with _model(case, exact=exact, path=path, kernel=kernel) as m:
         if path == 1:
@@ -194,7 +194,7 @@ def test_full_size_determinism_paths_and_budget(gm_full, gpu_available):
     with _model(c, exact=True, kernel=1) as m:
         m.integrate(10, dt=dt)
         assert np.array_equal(m.download("E"), ex), "marching and per-cell kernels differ in exact mode"
-    for kernel in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14):
+    for kernel in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17):
         with _model(c, kernel=kernel) as m:
             m.integrate(10, dt=dt)
             assert rel_l2(m.download("E"), e1) <= 1e-14, kernel
