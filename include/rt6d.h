@@ -187,8 +187,8 @@ int rt6d_fluxes(rt6d_ctx *ctx, double dt);
  * the last integrate (as the reference leaves them after integrate), or of
  * the last rt6d_fluxes call if it came later.  Velocities/lambdas/omegas:
  * calc_parameter outputs.  layout INTERIOR, FACES (velocities, fluxes) or
- * HALO (E only: the interior entries are written, ghost entries of the host
- * array are left as they were). */
+ * HALO (E, DE, MEANFL: the interior entries are written, ghost entries of the
+ * host array are left as they were). */
 int rt6d_download(rt6d_ctx *ctx, int field, int layout, double *out);
 
 /*

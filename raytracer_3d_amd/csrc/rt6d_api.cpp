@@ -870,9 +870,15 @@ int rt6d_download(rt6d_ctx *c, int field, int layout, double *out)
     }
     if (field == RT6D_FIELD_DE || field == RT6D_FIELD_MEANFL) {
         if ((r = require_ready(c))) return r;
-        if (layout != RT6D_LAYOUT_INTERIOR) return fail(RT6D_EINVAL, "rt6d_download: dE/meanfl are interior fields");
+        if (layout != RT6D_LAYOUT_INTERIOR && layout != RT6D_LAYOUT_HALO)
+            return fail(RT6D_EINVAL, "rt6d_download: layout %d for dE/meanfl", layout);
         if ((r = ensure_step_byproducts(c, false))) return r;
-        return download_typed(c, field == RT6D_FIELD_DE ? c->d_tdE : c->d_tmf, cells, out);
+        const void *src = field == RT6D_FIELD_DE ? c->d_tdE : c->d_tmf;
+        if (layout == RT6D_LAYOUT_INTERIOR) return download_typed(c, src, cells, out);
+        std::vector<double> tmp(cells);   // halo layout: the interior is written, the ghosts are left
+        if ((r = download_typed(c, src, cells, tmp.data()))) return r;
+        interior_to_halo(c->n, tmp.data(), out);
+        return 0;
     }
     if (field >= RT6D_FIELD_FLUX_X && field < RT6D_FIELD_FLUX_X + 6) {
         if ((r = require_ready(c))) return r;

@@ -2,7 +2,9 @@
 // (included by rt6d_kernels.h after rt6d_column.h, inside namespace rt6d).
 //
 // The column kernel's march (register window along m, carried m-face flux,
-// ZC z cells per thread sharing their inner z faces) with each thread owning
+// ZC z cells per thread sharing their inner z faces; the outer z planes are
+// read in the step that uses them, which keeps the binary64 register count
+// at four wavefronts per SIMD) with each thread owning
 // the pair of x cells (x, x+1), x odd, so that every access moves a pair: the
 // own pair, the pairs (x-2, x-1) and (x+2, x+3) for the x stencil, the pairs
 // of the four neighbouring rows for y and of the four outer z planes.  The
@@ -161,8 +163,6 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
         FLm[j] = Pair<T>{T(0), T(0)};
     }
     Pair<T> zq[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) zq[q] = az ? ldz(q, 1) : Pair<T>{T(0), T(0)};
     if (am) {
 #pragma unroll
         for (int j = 0; j < ZC; ++j) {
@@ -174,11 +174,11 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
     for (int m = 1; m <= nm; ++m) {
         const uint32_t q3 = Qkl + (uint32_t)(m - 1) * (uint32_t)nk * nl;
         const uint32_t q20 = P0 + (uint32_t)(m - 1) * nzkl;
-        Pair<T> wnx[ZC], znx[4];
+        Pair<T> wnx[ZC];
 #pragma unroll
         for (int j = 0; j < ZC; ++j) wnx[j] = ldm(j, m + 3);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) znx[q] = (az && m < nm) ? ldz(q, m + 1) : Pair<T>{T(0), T(0)};
+        for (int q = 0; q < 4; ++q) zq[q] = az ? ldz(q, m) : Pair<T>{T(0), T(0)};
 
         T FzA[ZC + 1], FzB[ZC + 1];
         if (az) {
@@ -262,8 +262,6 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
             wp1[j] = wp2[j];
             wp2[j] = wnx[j];
         }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) zq[q] = znx[q];
     }
 }
 

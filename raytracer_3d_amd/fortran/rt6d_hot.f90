@@ -221,3 +221,89 @@ subroutine diagnose
   print '(a,es14.6)', ' residual : ', d%residual
   deallocate(Ei, Mi, Di)
 end subroutine diagnose
+
+! Restart file name of this PE, 'restart_pe=' with the PE number in i9 and
+! blanks written as zeros, as for_src/restart.f90:7-8 names it.
+subroutine restart_file_name(fname)
+  use main_module
+  implicit none
+  character(len=80), intent(out) :: fname
+  integer :: i
+  write(fname, '(a,i9,a)') 'restart_pe=', my_pe, '.dta'
+  do i = 1, len_trim(fname)
+    if (fname(i:i) == ' ') fname(i:i) = '0'
+  enddo
+end subroutine restart_file_name
+
+! write_restart (for_src/restart.f90:1-30): the same four unformatted
+! records -- dimensions and itt, the PE bounds (two records), E, dE, meanfl
+! over their halo'd extents -- and the 'ritt' file.  E comes from the device;
+! dE and meanfl are the last step's by-products (zero before the first step).
+subroutine write_restart
+  use main_module
+  use rt6d_abi
+  implicit none
+  character(len=80) :: fname
+  integer :: io
+  call e_to_host
+  call rt6d_check(rt6d_download(rt6d_handle, RT6D_FIELD_DE, RT6D_LAYOUT_HALO, dE), 'write_restart/dE')
+  call rt6d_check(rt6d_download(rt6d_handle, RT6D_FIELD_MEANFL, RT6D_LAYOUT_HALO, meanfl), 'write_restart/meanfl')
+  call restart_file_name(fname)
+  print '(a,a)', ' writing to restart file ', trim(fname)
+  print '(a,i12)', ' at itt = ', itt
+  open(newunit=io, file=trim(fname), form='unformatted', status='unknown', err=10)
+  write(io, err=10) nx, ny, nz, nk, nl, nm, itt
+  write(io, err=10) xs_pe, xe_pe, ys_pe, ye_pe, zs_pe, ze_pe
+  write(io, err=10) ks_pe, ke_pe, ls_pe, le_pe, ms_pe, me_pe
+  write(io, err=10) E, dE, meanfl
+  close(io)
+  open(newunit=io, file='ritt', form='formatted', status='unknown')
+  write(io, *) itt
+  close(io)
+  return
+10 continue
+  print '(a)', ' Warning: error writing restart file'
+end subroutine write_restart
+
+! read_restart (for_src/restart.f90:35-93): nothing happens without a
+! restart file; dimensions and PE bounds must match; E goes to the device,
+! dE and meanfl stay on the host (the device recomputes its by-products at
+! the next step).
+subroutine read_restart
+  use main_module
+  implicit none
+  character(len=80) :: fname
+  logical :: file_exists
+  integer :: io, nx_, ny_, nz_, nk_, nl_, nm_
+  integer :: b(12), bb(12)
+  call restart_file_name(fname)
+  inquire(file=trim(fname), exist=file_exists)
+  if (.not. file_exists) then
+    print '(a,a)', ' found no restart file ', trim(fname)
+    return
+  endif
+  print '(a,a)', ' reading restart file ', trim(fname)
+  open(newunit=io, file=trim(fname), form='unformatted', status='old', err=10)
+  read(io, err=10) nx_, ny_, nz_, nk_, nl_, nm_, itt
+  if (nx /= nx_ .or. ny /= ny_ .or. nz /= nz_ .or. nk /= nk_ .or. nl /= nl_ .or. nm /= nm_) then
+    print *, ' read from restart dimensions: ', nx_, ny_, nz_, nk_, nl_, nm_
+    print *, ' does not match dimensions   : ', nx, ny, nz, nk, nl, nm
+    call halt_stop(' read_restart')
+  endif
+  read(io, err=10) b(1:6)
+  read(io, err=10) b(7:12)
+  bb = [xs_pe, xe_pe, ys_pe, ye_pe, zs_pe, ze_pe, ks_pe, ke_pe, ls_pe, le_pe, ms_pe, me_pe]
+  if (any(b /= bb)) then
+    print *, ' read from restart PE boundaries: ', b
+    print *, ' which does not match           : ', bb
+    call halt_stop(' read_restart')
+  endif
+  read(io, err=10) E, dE, meanfl
+  close(io)
+  print *, ' itt = ', itt
+  call e_to_device
+  return
+10 continue
+  print '(a)', ' Warning: error reading restart file'
+  call halt_stop(' read_restart')
+end subroutine read_restart

@@ -250,3 +250,26 @@ def test_config0_packet_full_size(gpu_available):
     with _model(c, precision=rt6d.FP32) as m:
         m.integrate(200, dt=dt)
         assert rel_l2(m.download("E"), ex) <= TOL32
+
+
+def test_beyond_32bit_offsets_uses_the_per_cell_kernel(gpu_available):
+    """48^3 x 16^3 (452,984,832 cells, 3.6 GB per binary64 field): the column
+    kernels address arrays with 32-bit byte offsets, so the binary64 step falls
+    back to the per-cell kernel while FP32 (1.8 GB fields) keeps the column
+    kernel; both stay finite, the budget closes, and they agree to FP32 accuracy."""
+    spec = cases.member_spec(11, 2, n=(48, 48, 48, 16, 16, 16))
+    with rt6d.Model.from_case(spec) as m64:
+        m64.calc_parameter()
+        assert m64.homogeneous and m64.query(rt6d.Q_CELLS) == 452984832
+        dt = m64.cfl_dt(0.5)
+        m64.integrate(4, dt=dt)
+        d = m64.diagnose(dt=dt)
+        e64 = m64.download("E")
+    terms = [d["eint"], d["mflint"]] + [d[k] for k in d if k.startswith("bfl_")]
+    assert abs(d["residual"]) <= 1e-11 * sum(abs(t) for t in terms)
+    assert np.all(np.isfinite(e64))
+    with rt6d.Model.from_case(spec, precision=rt6d.FP32) as m32:
+        m32.calc_parameter()
+        m32.integrate(4, dt=dt)
+        e32 = m32.download("E")
+    assert rel_l2(e32, e64) <= TOL32
