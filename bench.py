@@ -168,20 +168,21 @@ def main():
     cells = case.cells
     workload = "gm_24x24x24_16x16x16"
 
-    def run(precision, path, kernel=0):
-        m = rt6d.Model.from_case(case, precision=precision, path=path, kernel=kernel)
+    def run(precision, path, kernel=0, c=None):
+        c = case if c is None else c
+        m = rt6d.Model.from_case(c, precision=precision, path=path, kernel=kernel)
         m.calc_parameter()
         dt = m.cfl_dt(0.5)
         wall, ev_ms = _time_steps(m, dt, args.steps, args.warmup)
         res = {
-            "value": cells * args.steps / wall,
+            "value": c.cells * args.steps / wall,
             "ms_per_step": 1e3 * wall / args.steps,
             "launch_ms": ev_ms / args.steps,
             "bytes_per_cell": m.bytes_per_cell,
             "homogeneous": m.homogeneous,
             "dt": dt,
         }
-        res["achieved_GBs"] = res["bytes_per_cell"] * cells / (res["launch_ms"] * 1e-3) / 1e9
+        res["achieved_GBs"] = res["bytes_per_cell"] * c.cells / (res["launch_ms"] * 1e-3) / 1e9
         m.close()
         return res
 
@@ -197,6 +198,14 @@ def main():
         extras["fp32_general"] = run(rt6d.FP32, 1)
         _log("binary64 step, per-cell kernel")
         extras["fp64_percell"] = run(rt6d.FP64, 0, kernel=1)
+        _log("32^3 x 16^3 (134M cells): binary64 step and FP32 form")
+        big = cases.gm_case(n=(32, 32, 32, 16, 16, 16))
+        extras["fp64_32cube"] = run(rt6d.FP64, 0, c=big)
+        extras["fp32_32cube"] = run(rt6d.FP32, 0, c=big)
+        extras["fp32_32cube"]["speedup_vs_fp64"] = extras["fp32_32cube"]["value"] / extras["fp64_32cube"]["value"]
+        for k in ("fp64_32cube", "fp32_32cube"):
+            extras[k]["cells"] = big.cells
+        del big
         from raytracer_3d_amd import serve
         for prec, tag in ((rt6d.FP64, "ensemble96_fp64"), (rt6d.FP32, "ensemble96_fp32")):
             _log(f"batched ensemble, 96 members x {args.ensemble_steps} steps ({tag[-4:]})")
