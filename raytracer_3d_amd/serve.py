@@ -11,7 +11,13 @@ per HIP graph replay).  Every tick advances each ensemble by
 min(chunk, steps left of its members); members whose steps are done leave
 their ensemble, their result (E, and the energy budget when asked) is read,
 their memory freed, and queued requests take the room.  State stays on the
-device from admission to completion.
+device from admission to completion, unless preemption is on: then a request
+that has waited long enough at the head of the queue may preempt a resident
+one with many steps left. The resident one is checkpointed (its E, in memory
+or as a restart file in the reference's layout, raytracer_3d_amd.restart),
+its device memory freed, and it queues again right behind the newcomer.
+When it is re-admitted it is rebuilt, resumes from the checkpoint and
+finishes its remaining steps with the same result.
 
     python -m raytracer_3d_amd.serve --rate 40 --duration 120 --budget-gb 200
 
@@ -58,6 +64,8 @@ class Request:
     started: float = math.nan
     done: float = math.nan
     result: dict | None = None
+    preemptions: int = 0
+    resume: dict | None = None   # checkpoint of a preempted request: steps done / left, E or restart path
 
     @property
     def cells(self) -> int:
@@ -105,11 +113,20 @@ def peak_bytes(n, precision) -> int:
 
 
 def build_member(req: Request, cfl: float = 0.5):
-    """Device-resident integration of a request (calc_parameter done, E set)."""
+    """Device-resident integration of a request (calc_parameter done, E set;
+    a preempted request resumes from its checkpoint)."""
     spec = cases.member_spec(req.seed, req.member, n=req.n)
     m = rt6d.Model.from_case(spec, precision=req.precision)
     m.calc_parameter()
     m.dt = m.cfl_dt(cfl)
+    if req.resume is not None:
+        if "path" in req.resume:
+            from . import restart
+            itt = restart.load(m, req.resume["path"])
+            if itt != req.resume["done"]:
+                raise RuntimeError(f"request {req.rid}: restart file at step {itt}, expected {req.resume['done']}")
+        else:
+            m.upload_E(req.resume["E"])
     return m
 
 
@@ -117,8 +134,16 @@ class Server:
     """Continuous batching over one ensemble per precision."""
 
     def __init__(self, budget_bytes: float, chunk: int = 8, max_members: int = 1024, keep_E: bool = False,
-                 diagnose: bool = False, tune: bool = True, tune_min_members: int = 8):
+                 diagnose: bool = False, tune: bool = True, tune_min_members: int = 8,
+                 preempt_after: float | None = None, checkpoint_dir: str | None = None):
+        """preempt_after: seconds a request may wait at the head of the queue
+        for HBM room before it preempts a resident request (None: never).
+        checkpoint_dir: write preempted requests' state as restart files there
+        (else keep E in host memory)."""
         self.budget = float(budget_bytes)
+        self.preempt_after = preempt_after
+        self.checkpoint_dir = checkpoint_dir
+        self.preemptions = 0
         self.chunk = int(chunk)
         self.max_members = int(max_members)
         self.keep_E = keep_E
@@ -173,8 +198,15 @@ class Server:
                     r = waiting[0]
                     need = peak_bytes(r.n, r.precision)
                     with lock:
-                        if self.used + need > self.budget and (building or any(active.values())):
+                        fits = not (self.used + need > self.budget and (building or any(active.values())))
+                    if not fits:
+                        victim = self._victim(r, active, now())
+                        if victim is None:
                             break
+                        self._preempt(victim, active, lock)
+                        waiting.insert(1, victim[1])
+                        continue
+                    with lock:
                         self.used += need
                         self.peak_used = max(self.peak_used, self.used)
                     waiting.popleft()
@@ -193,8 +225,9 @@ class Server:
                     with lock:
                         self.used += resident_bytes(r.n, r.precision) - peak_bytes(r.n, r.precision)
                     slot = self.ens[r.precision].add(m)
-                    r.started = now()
-                    active[r.precision][slot] = [r, m, r.steps]
+                    if not np.isfinite(r.started):
+                        r.started = now()
+                    active[r.precision][slot] = [r, m, r.resume["left"] if r.resume else r.steps]
                 if not any(active.values()):
                     if building:
                         r, m, err = ready.get()   # wait for the set-up thread
@@ -231,7 +264,7 @@ class Server:
                     for slot in finished:
                         r, m, _ = act.pop(slot)
                         e.remove(slot)
-                        res = {"steps": m.query(rt6d.Q_STEPS), "dt": m.dt}
+                        res = {"steps": m.query(rt6d.Q_STEPS) + (r.resume["done"] if r.resume else 0), "dt": m.dt}
                         if self.diagnose:
                             d = m.diagnose()
                             res.update(eint=d["eint"], residual=d["residual"])
@@ -251,11 +284,47 @@ class Server:
             th.join(timeout=30)
         return now()
 
+    def _victim(self, r: Request, active, t: float):
+        """The resident request that the queue head r preempts, or None: r has
+        waited preempt_after seconds, neither was preempted before, and the
+        victim has more than two chunks of steps left (the one with the most)."""
+        if self.preempt_after is None or r.preemptions or t - r.arrival < self.preempt_after:
+            return None
+        best = None
+        for prec, act in active.items():
+            for slot, v in act.items():
+                q, _, left = v
+                if q.preemptions or left <= 2 * self.chunk:
+                    continue
+                if best is None or left > best[2]:
+                    best = (prec, q, left, slot)
+        return None if best is None else (best[0], best[1], best[3])
+
+    def _preempt(self, victim, active, lock):
+        prec, q, slot = victim
+        _, m, left = active[prec].pop(slot)
+        self.ens[prec].remove(slot)
+        done = q.steps - left
+        if self.checkpoint_dir:
+            import os
+            from . import restart
+            path = os.path.join(self.checkpoint_dir, f"restart_request{q.rid:06d}.dta")
+            restart.save(m, path, done, ritt=False)
+            q.resume = {"done": done, "left": left, "path": path}
+        else:
+            q.resume = {"done": done, "left": left, "E": m.download("E")}
+        m.close()
+        with lock:
+            self.used -= resident_bytes(q.n, q.precision)
+        q.preemptions += 1
+        self.preemptions += 1
+
 
 def run_alone(req: Request, precision=None):
     """The request run on its own (plain single-integration steps); returns
     (seconds, E)."""
-    r = dataclasses.replace(req, precision=req.precision if precision is None else precision)
+    r = Request(req.rid, req.arrival, req.member, req.n, req.steps,
+                req.precision if precision is None else precision, req.seed)   # none of the server's fields
     m = build_member(r)
     m.synchronize()
     t0 = time.perf_counter()
@@ -278,10 +347,10 @@ def fifo_latencies(arrivals, service):
 
 
 def serve_benchmark(rate=40.0, duration=120.0, budget_gb=200.0, chunk=8, baseline_seconds=20.0, seed=4242,
-                    log=None):
+                    log=None, preempt_after=None, checkpoint_dir=None):
     reqs = request_stream(rate, duration, seed=seed)
     work = sum(r.cells * r.steps for r in reqs)
-    srv = Server(budget_gb * 2**30, chunk=chunk)
+    srv = Server(budget_gb * 2**30, chunk=chunk, preempt_after=preempt_after, checkpoint_dir=checkpoint_dir)
     try:
         wall = srv.run(reqs, log=log)
     finally:
@@ -293,6 +362,7 @@ def serve_benchmark(rate=40.0, duration=120.0, budget_gb=200.0, chunk=8, baselin
         "mean_latency_s": float(lat.mean()), "device_ms": srv.device_ms, "ticks": srv.ticks,
         "peak_hbm_gib": srv.peak_used / 2**30,
         "fp32_share": float(np.mean([r.precision == rt6d.FP32 for r in reqs])),
+        "preemptions": srv.preemptions,
     }
     # the same requests one at a time with the plain binary64 step (bounded
     # sample): the step loop alone, and the whole request (set-up included, as
@@ -371,9 +441,13 @@ def main(argv=None):
     ap.add_argument("--chunk", type=int, default=8)
     ap.add_argument("--baseline-seconds", type=float, default=20.0)
     ap.add_argument("--seed", type=int, default=4242)
+    ap.add_argument("--preempt-after", type=float, default=None,
+                    help="seconds a request may wait for HBM room before it preempts a resident one")
+    ap.add_argument("--checkpoint-dir", default=None, help="write preempted state as restart files there")
     a = ap.parse_args(argv)
     out = serve_benchmark(a.rate, a.duration, a.budget_gb, a.chunk, a.baseline_seconds, a.seed,
-                          log=lambda s: print("[serve] " + s, file=sys.stderr, flush=True))
+                          log=lambda s: print("[serve] " + s, file=sys.stderr, flush=True),
+                          preempt_after=a.preempt_after, checkpoint_dir=a.checkpoint_dir)
     print(json.dumps(out), flush=True)
     return 0
 

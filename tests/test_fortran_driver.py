@@ -116,3 +116,47 @@ def test_driver_restart_resumes_bit_for_bit(tmp_path, gpu_available):
     assert "reading restart file" in r.stdout
     assert np.array_equal(np.fromfile(tmp_path / "E5.out", dtype="<f8"), E5)
     assert list(np.frombuffer(_fortran_records(tmp_path / "restart_pe=000000000.dta")[0], "<i4"))[-1] == 5
+
+
+@needs_driver
+@pytest.mark.gpu
+def test_restart_files_cross_between_fortran_and_python(tmp_path, gpu_available):
+    """A restart file written by the Fortran driver reads in Python
+    (raytracer_3d_amd.restart) and resumes a device context; one written from
+    Python resumes the Fortran driver. Both continuations equal the
+    uninterrupted run bit for bit."""
+    from raytracer_3d_amd import restart, rt6d
+    entry = golden_index()[0]
+    case = golden_case(entry)
+    cf3, cf2 = str(tmp_path / "c3.case"), str(tmp_path / "c2.case")
+    cases.write_case_file(case, cf3, nsteps=3)
+    cases.write_case_file(case, cf2, nsteps=2)
+    with rt6d.Model.from_case(case) as m:
+        m.calc_parameter()
+        m.integrate(3)
+        E3 = m.download("E")
+        m.integrate(2)
+        E5 = m.download("E")
+    # Fortran writes, Python reads and resumes
+    r = subprocess.run([DRIVER, cf3, "E3.out", "0", "fp64", "restart"], capture_output=True, text=True,
+                       timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    f = tmp_path / restart.restart_file_name(0)
+    got = restart.read_restart(str(f), case.n)
+    assert got["itt"] == 3 and np.array_equal(got["E"], E3)
+    with rt6d.Model.from_case(case) as m:
+        m.calc_parameter()
+        assert restart.load(m, str(f)) == 3
+        m.integrate(2)
+        assert np.array_equal(m.download("E"), E5)
+        restart.save(m, str(f), 5)      # Python writes at step 5 over the Fortran file
+    got = restart.read_restart(str(f), case.n)
+    assert got["itt"] == 5 and np.array_equal(got["E"], E5)
+    # Python writes at step 3, Fortran reads and runs the last 2 steps
+    restart.write_restart(str(f), case.n, 3, E3)
+    r = subprocess.run([DRIVER, cf2, "E5.out", "0", "fp64", "restart"], capture_output=True, text=True,
+                       timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "reading restart file" in r.stdout
+    assert np.array_equal(np.fromfile(tmp_path / "E5.out", dtype="<f8"), E5)
+

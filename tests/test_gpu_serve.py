@@ -48,6 +48,27 @@ def test_each_result_equals_the_request_alone(gpu_available):
     _check_alone(reqs)
 
 
+@pytest.mark.parametrize("to_disk", [False, True], ids=["memory", "restart_files"])
+def test_preempted_requests_resume_to_the_same_result(tmp_path, to_disk, gpu_available):
+    """With room for two members and preemption at once, long requests are
+    checkpointed, evicted and resumed; every result still equals the request
+    run alone, and its step count is whole."""
+    reqs = _requests(8, steps=(100, 200), spacing=0.0, seed=7)   # all queued at once
+    budget = 2 * max(serve.peak_bytes(r.n, r.precision) for r in reqs)
+    srv = serve.Server(budget, chunk=4, keep_E=True, tune=False, preempt_after=0.0,
+                       checkpoint_dir=str(tmp_path) if to_disk else None)
+    try:
+        srv.run(reqs)
+        assert srv.preemptions >= 1
+        assert srv.used == pytest.approx(0.0, abs=1.0)
+    finally:
+        srv.close()
+    assert any(r.preemptions for r in reqs)
+    if to_disk:
+        assert list(tmp_path.glob("restart_request*.dta"))
+    _check_alone(reqs)
+
+
 def test_tuned_mix_equals_alone(gpu_available):
     reqs = _requests(10, steps=(300, 400), spacing=0.0)
     for r in reqs:
