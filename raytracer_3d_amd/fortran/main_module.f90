@@ -23,6 +23,7 @@ module main_module
   logical :: enable_lower_reflection = .true.
   integer :: itt = 0
   real*8 :: runlen = 0d0, snapint = 0d0
+  logical :: enable_write_6D_single = .true.   ! snap6D.cdf next to snap3D.cdf (main_module.f90:122)
 
   ! host mirrors of the state (halo'd: (1-onx : n+onx) per axis)
   real*8, allocatable :: E(:,:,:,:,:,:), dE(:,:,:,:,:,:), meanfl(:,:,:,:,:,:)

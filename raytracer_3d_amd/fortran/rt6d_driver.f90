@@ -6,7 +6,8 @@
 !   CASE       case file (config_case.f90)
 !   OUT        if given and not '-', the final interior E is written there
 !              as a raw float64 stream
-!   SNAPSTEPS  diagnose every SNAPSTEPS steps and at step 0 (0: never)
+!   SNAPSTEPS  diagnose every SNAPSTEPS steps and at step 0 (0: never),
+!              writing snap3D.cdf and snap6D.cdf in the working directory
 !   fp32       run the FP32 form
 !   restart    read_restart after calc_parameter (main.f90:49) and
 !              write_restart after the loop (main.f90:81), in the working
@@ -50,6 +51,7 @@ program main
 
   itt0 = itt
   enditt = itt + int(runlen/dt + 0.5d0)
+  if (snapsteps > 0) call init_diag
   print '(a,i8,a,i8,a,es12.5,a)', ' integrating from step ', itt, ' to ', enditt, ' with dt = ', dt, ' s'
   call system_clock(c0, crate)
   do while (itt < enditt)

@@ -219,6 +219,7 @@ subroutine diagnose
   print '(a,2es14.6)', ' by flux at min/max l  : ', b(9), b(10)
   print '(a,2es14.6)', ' by flux at min/max m  : ', b(11), b(12)
   print '(a,es14.6)', ' residual : ', d%residual
+  call write_snapshots(d, Ei, Mi, Di)   ! after init_diag; a no-op without it
   deallocate(Ei, Mi, Di)
 end subroutine diagnose
 

@@ -38,8 +38,9 @@ def test_driver_matches_reference(entry, tmp_path, gpu_available):
     cf = str(tmp_path / "c.case")
     of = str(tmp_path / "E.out")
     cases.write_case_file(case, cf, nsteps=int(g["nsteps"]))
-    r = subprocess.run([DRIVER, cf, of, "2"], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([DRIVER, cf, of, "2"], capture_output=True, text=True, timeout=120, cwd=tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
+    assert (tmp_path / "snap3D.cdf").exists() and (tmp_path / "snap6D.cdf").exists()
     E = np.fromfile(of, dtype="<f8")
     assert rel_l2(E, g["EN"]) <= 1e-12
     res = [float(x) for x in re.findall(r"residual :\s*(\S+)", r.stdout)]
@@ -160,3 +161,33 @@ def test_restart_files_cross_between_fortran_and_python(tmp_path, gpu_available)
     assert "reading restart file" in r.stdout
     assert np.array_equal(np.fromfile(tmp_path / "E5.out", dtype="<f8"), E5)
 
+
+@needs_driver
+@pytest.mark.gpu
+def test_driver_snapshots_equal_the_python_hosts(tmp_path, gpu_available):
+    """With diagnose every 3 steps the Fortran driver writes snap3D.cdf and
+    snap6D.cdf (fortran/rt6d_netcdf.f90); the Python host's time loop
+    (raytracer_3d_amd.driver) writes the same files byte for byte, and they
+    read back with scipy in the reference's schema."""
+    from scipy.io import netcdf_file
+    from raytracer_3d_amd import driver
+    entry = golden_index()[0]
+    case = golden_case(entry)
+    cf = str(tmp_path / "c.case")
+    cases.write_case_file(case, cf, nsteps=7)
+    fdir, pdir = tmp_path / "fortran", tmp_path / "python"
+    fdir.mkdir()
+    r = subprocess.run([DRIVER, cf, "-", "3", "fp64"], capture_output=True, text=True, timeout=120, cwd=fdir)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "time steps in file snap3D.cdf :      3" in r.stdout
+    res = driver.run(case, nsteps=7, snapint=3 * case.dt * (1 + 1e-9), directory=str(pdir))
+    assert res["snapshots"] == 3
+    for name in ("snap3D.cdf", "snap6D.cdf"):
+        a, b = (fdir / name).read_bytes(), (pdir / name).read_bytes()
+        assert len(a) == len(b), name
+        assert a == b, f"{name}: first difference at byte {next(i for i in range(len(a)) if a[i] != b[i])}"
+    with netcdf_file(str(fdir / "snap3D.cdf"), "r", mmap=False) as f:
+        np.testing.assert_array_equal(f.variables["Time"][:], np.array([0, 3, 6]) * case.dt)
+        assert f.variables["E"].dimensions == ("Time", "zt", "yt", "xt")
+    with netcdf_file(str(fdir / "snap6D.cdf"), "r", mmap=False) as f:
+        assert np.array_equal(f.variables["E"][2].reshape(-1), res["E"])
