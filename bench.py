@@ -73,9 +73,10 @@ def _pe_grid(n, npes):
     return grid if left == 1 else None
 
 
-def cpu_baseline(case, dt, steps):
+def cpu_baseline(case, dt, steps, ranks=(16, 8, 4, 2), label="24^3x16^3"):
     """The reference Fortran (oracle/_ref/ref_harness) on the host cores,
-    MPI ranks over a 6D PE grid; returns the cpu_baseline object."""
+    MPI ranks over a 6D PE grid (the first count of `ranks` that the host
+    and the grid allow); returns the cpu_baseline object."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from raytracer_3d_amd import cases
@@ -84,7 +85,7 @@ def cpu_baseline(case, dt, steps):
         return None
     ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     tries = []
-    for npes in (16, 8, 4, 2):
+    for npes in ranks:
         if npes <= ncpu:
             g = _pe_grid(case.n, npes)
             if g:
@@ -103,7 +104,7 @@ def cpu_baseline(case, dt, steps):
                     "unit": "cell-updates/s",
                     "cores": npes,
                     "kind": "reference",
-                    "sample": f"{steps} integrate steps of the same 24^3x16^3 binary64 workload, "
+                    "sample": f"{steps} integrate steps of the same {label} binary64 workload, "
                               f"MPI_Wtime around the loop, {npes} rank(s) PE grid {g}, host nproc {ncpu}",
                     "seconds": secs,
                 }
@@ -206,6 +207,13 @@ def main():
         for k in ("fp64_32cube", "fp32_32cube"):
             extras[k]["cells"] = big.cells
         del big
+        _log("config-0 packet 32x4x32x16x4x16 (4.2M cells): binary64 step, reference Fortran on 4 ranks")
+        c0 = cases.packet_case()
+        extras["config0_packet_fp64"] = run(rt6d.FP64, 0, c=c0)
+        extras["config0_packet_fp64"]["cells"] = c0.cells
+        if not args.no_cpu_baseline:
+            extras["config0_packet_fp64"]["cpu_baseline_np4"] = cpu_baseline(
+                c0, extras["config0_packet_fp64"]["dt"], 20, ranks=(4, 2), label="config-0 packet 32x4x32x16x4x16")
         from raytracer_3d_amd import serve
         for prec, tag in ((rt6d.FP64, "ensemble96_fp64"), (rt6d.FP32, "ensemble96_fp32")):
             _log(f"batched ensemble, 96 members x {args.ensemble_steps} steps ({tag[-4:]})")
