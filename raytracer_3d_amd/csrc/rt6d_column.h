@@ -38,6 +38,11 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t mk_rsrc(const void *p, uint64_
                                              0x00020000);
 }
 
+// Cache policy of the step kernels' stores of the new E: non-temporal (the
+// values are not read again in this step), which keeps L2 for the inputs
+// (measured 1-2% faster in binary64 at 24^3 x 16^3).
+constexpr int kStoreNT = 2;
+
 template <typename T>
 __device__ __forceinline__ T bld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
 template <>
@@ -56,12 +61,12 @@ template <>
 __device__ __forceinline__ void bst<double>(double v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
 {
     using v2i = int __attribute__((ext_vector_type(2)));
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2i, v), r, (int)voff, (int)soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2i, v), r, (int)voff, (int)soff, kStoreNT);
 }
 template <>
 __device__ __forceinline__ void bst<float>(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
 {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, (int)voff, (int)soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, (int)voff, (int)soff, kStoreNT);
 }
 
 // Wave-uniform read of a table that no kernel writes while it runs: through

@@ -52,7 +52,7 @@ __device__ __forceinline__ void bst2<double>(Pair<double> v, __amdgpu_buffer_rsr
     v2d d;
     d.x = v.a;
     d.y = v.b;
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4i, d), r, (int)voff, (int)soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4i, d), r, (int)voff, (int)soff, kStoreNT);
 }
 template <>
 __device__ __forceinline__ void bst2<float>(Pair<float> v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
@@ -62,7 +62,7 @@ __device__ __forceinline__ void bst2<float>(Pair<float> v, __amdgpu_buffer_rsrc_
     v2f f;
     f.x = v.a;
     f.y = v.b;
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2i, f), r, (int)voff, (int)soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2i, f), r, (int)voff, (int)soff, kStoreNT);
 }
 
 // Column configuration of the pair kernel: the padded plane counts pairs.
