@@ -3,9 +3,13 @@
 //
 // Every thread owns one (x,y) column of ZC consecutive z cells at one (k,l)
 // and marches it through m = 1..nm with no LDS and no barriers:
-//  * E along m sits in a register window per cell (E(m-1..m+2), E(m+3) in
-//    flight): each E value is read from HBM once and the m-face flux of step
-//    m is carried as the left face of step m+1;
+//  * E along m sits in a register window per cell (E(m-1..m+1), and E(m+2)
+//    loaded at the start of step m for its m face; E(m+3) one step ahead
+//    with four z cells per thread): each E value is read from HBM once and
+//    the m-face flux of step m is carried as the left face of step m+1.
+//    Loading two steps ahead rather than three keeps less of E waiting in L2
+//    for the neighbours' loads (binary64 pair kernel at 24^3 x 16^3: 717
+//    instead of 877 MB fetched per step, at the same speed);
 //  * the ZC cells share their ZC-1 inner z faces; the four outer z
 //    neighbours are read one plane ahead;
 //  * x and y neighbours are plain loads (L1/L2 hits: the neighbouring lanes
@@ -201,12 +205,16 @@ __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, con
         const int mu = zmir[q] ? nm + 1 - mm : mm;
         return bld<T>(rE, oc, base0 + (uint32_t)(zsh[q] * (int)planeB) + (uint32_t)(mu - 1) * mB);
     };
+    // own-plane look-ahead: E(m+2) loaded at the start of step m, or with
+    // four z cells per thread (whose first m face comes early in the step)
+    // E(m+3) one step ahead
+    constexpr int LA = ZC >= 4 ? 3 : 2;
     T wm1[ZC], w0[ZC], wp1[ZC], wp2[ZC], FLm[ZC];
 #pragma unroll
     for (int j = 0; j < ZC; ++j) {
         w0[j] = ldm(j, 1);
         wp1[j] = ldm(j, 2);
-        wp2[j] = ldm(j, 3);
+        if constexpr (LA == 3) wp2[j] = ldm(j, 3);
         wm1[j] = w0[j];
         FLm[j] = T(0);
     }
@@ -237,9 +245,12 @@ __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, con
     for (int m = 1; m <= nm; ++m) {
         const uint32_t q3 = Qkl + (uint32_t)(m - 1) * (uint32_t)nk * nl;
         const uint32_t q20 = P0 + (uint32_t)(m - 1) * nzkl;
-        T wnx[ZC], znx[4];
+        T znx[4], wnx[ZC];
 #pragma unroll
-        for (int j = 0; j < ZC; ++j) wnx[j] = ldm(j, m + 3);
+        for (int j = 0; j < ZC; ++j) {
+            if constexpr (LA == 3) wnx[j] = ldm(j, m + 3);
+            else wp2[j] = ldm(j, m + 2);
+        }
 #pragma unroll
         for (int q = 0; q < 4; ++q) znx[q] = (az && m < nm) ? ldz(q, m + 1) : T(0);
 
@@ -381,7 +392,7 @@ __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, con
             wm1[j] = w0[j];
             w0[j] = wp1[j];
             wp1[j] = wp2[j];
-            wp2[j] = wnx[j];
+            if constexpr (LA == 3) wp2[j] = wnx[j];
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) zq[q] = znx[q];

@@ -153,12 +153,13 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
         const int mu = zmir[q] ? nm + 1 - mm : mm;
         return bld2<T>(rE, oc, base0 + (uint32_t)(zsh[q] * (int)planeB) + (uint32_t)(mu - 1) * mB);
     };
+    constexpr int LA = ZC >= 4 ? 3 : 2;   // own-plane look-ahead, as in the column kernel
     Pair<T> wm1[ZC], w0[ZC], wp1[ZC], wp2[ZC], FLm[ZC];
 #pragma unroll
     for (int j = 0; j < ZC; ++j) {
         w0[j] = ldm(j, 1);
         wp1[j] = ldm(j, 2);
-        wp2[j] = ldm(j, 3);
+        if constexpr (LA == 3) wp2[j] = ldm(j, 3);
         wm1[j] = w0[j];
         FLm[j] = Pair<T>{T(0), T(0)};
     }
@@ -176,7 +177,10 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
         const uint32_t q20 = P0 + (uint32_t)(m - 1) * nzkl;
         Pair<T> wnx[ZC];
 #pragma unroll
-        for (int j = 0; j < ZC; ++j) wnx[j] = ldm(j, m + 3);
+        for (int j = 0; j < ZC; ++j) {
+            if constexpr (LA == 3) wnx[j] = ldm(j, m + 3);
+            else wp2[j] = ldm(j, m + 2);   // E(m+2), for this step's m face
+        }
 #pragma unroll
         for (int q = 0; q < 4; ++q) zq[q] = az ? ldz(q, m) : Pair<T>{T(0), T(0)};
 
@@ -260,7 +264,7 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
             wm1[j] = w0[j];
             w0[j] = wp1[j];
             wp1[j] = wp2[j];
-            wp2[j] = wnx[j];
+            if constexpr (LA == 3) wp2[j] = wnx[j];
         }
     }
 }
