@@ -203,6 +203,10 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
 #pragma unroll
         for (int j = 0; j < ZC; ++j) {
             const uint32_t Pm = P0 + (uint32_t)j + (uint32_t)(m - 1) * nzkl;
+            // raised wave priority while this cell pair's neighbour loads issue,
+            // so that they leave early and other waves' arithmetic covers them
+            // (binary64 at 24^3 x 16^3: 0.396 instead of 0.417 ms)
+            __builtin_amdgcn_s_setprio(1);
             const uint32_t pb = Pm * planeB;
             const uint32_t q2 = q20 + (uint32_t)j;
             const Pair<T> e = w0[j];
@@ -222,6 +226,7 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
             if (ay) {
                 const Pair<T> a0 = bld2<T>(rE, oy[0], pb), a1 = bld2<T>(rE, oy[1], pb);
                 const Pair<T> a3 = bld2<T>(rE, oy[2], pb), a4 = bld2<T>(rE, oy[3], pb);
+                __builtin_amdgcn_s_setprio(0);
                 const T v0 = sld(p.hom.vy3, q2), vi = sld(p.hom.vy3, p.hom.n4 + q2), vb = sld(p.hom.vy3, 2 * p.hom.n4 + q2);
                 const T vL = y == 1 ? v0 : vi, vR = y == ny ? vb : vi;
                 const T FLa = face_flux<T, EXACT>(vL, dtv, dyl, cyl, a0.a, a1.a, e.a, a3.a);
@@ -231,6 +236,7 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
                 dEa = div_term<T, EXACT>(dEa, FRa, FLa, dyr, ryy);
                 dEb = div_term<T, EXACT>(dEb, FRb, FLb, dyr, ryy);
             }
+            if (!ay) __builtin_amdgcn_s_setprio(0);
             if (az) {
                 const int zc = z0 + j;
                 const double dzc = sld(p.ax.dx[2], zc), rzc = sld(p.ax.rdx[2], zc);
