@@ -65,10 +65,8 @@ extern "C" {
 #define RT6D_OPT_BLOCK 3     /* threads per block of the per-cell step kernel (64..1024) */
 #define RT6D_OPT_KERNEL 4    /* step kernel: 0 auto (default: the column kernel with two z
                                 cells per thread when nz is even, with two x cells per
-                                thread on the binary64 homogeneous path when nx is even,
-                                and on the binary32 one (four z cells per thread) when
-                                nx is even, nz divisible by 4 and (nx/2)*ny by 64,
-                                with the y neighbours staged in LDS on the general path);
+                                thread on the homogeneous path when nx is even, with
+                                the y neighbours staged in LDS on the general path);
                                 1 per-cell kernel;
                                 2/3 LDS marching kernel, 1/2 z cells per thread (nx*ny <= 1024);
                                 4/5/6 column kernel, 1/2/4 z cells per thread;

@@ -1172,7 +1172,7 @@ int ens_rebuild_t(rt6d_ensemble *e)
             M.p[0] = step_ptrs<T>(c, c->d_E[a], c->d_E[a ^ 1], e->dt[i]);
             M.p[1] = step_ptrs<T>(c, c->d_E[a ^ 1], c->d_E[a], e->dt[i]);
             const bool full = c->homog ? is_full<true>(c->g) : is_full<false>(c->g);
-            const bool x2 = sizeof(T) == 8 && colx2_fits(c->g, c->homog, zc);
+            const bool x2 = colx2_fits(c->g, c->homog, zc);
             v = (c->homog ? 1 : 0) | (zc == 2 ? 2 : 0) | (full ? 4 : 0) | (x2 ? 8 : 0);
             M.variant = (uint32_t)v;
             e->cur0[i] = a;

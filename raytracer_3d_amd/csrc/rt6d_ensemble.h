@@ -25,7 +25,7 @@ struct EnsMember {
     ColCfg cc;
     ColX2Cfg cx;        // pair-kernel configuration (variants with bit 3)
     uint32_t variant;   // bit 0: homogeneous tables; bit 1: two z cells per thread; bit 2: all axes
-                        // active; bit 3: two x cells per thread (binary64, homogeneous, nx even)
+                        // active; bit 3: two x cells per thread (homogeneous, nx even)
     uint32_t pad;
 };
 

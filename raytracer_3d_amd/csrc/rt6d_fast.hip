@@ -82,7 +82,7 @@ hipError_t launch_ensemble(const EnsMember<T> *mem, const EnsTask *tasks, const 
         case 5: ens_launch_variant<T, true, 1, true>(mem, tv, cnt[v], parity, persist_blocks, cv, s); break;
         case 6: ens_launch_variant<T, false, 2, true>(mem, tv, cnt[v], parity, persist_blocks, cv, s); break;
         case 7: ens_launch_variant<T, true, 2, true>(mem, tv, cnt[v], parity, persist_blocks, cv, s); break;
-        // pair kernel: binary64 homogeneous members only (bit 0 set)
+        // pair kernel: homogeneous members only (bit 0 set)
         case 9: ens_launch_variant<T, true, 1, false, true>(mem, tv, cnt[v], parity, persist_blocks, cv, s); break;
         case 11: ens_launch_variant<T, true, 2, false, true>(mem, tv, cnt[v], parity, persist_blocks, cv, s); break;
         case 13: ens_launch_variant<T, true, 1, true, true>(mem, tv, cnt[v], parity, persist_blocks, cv, s); break;

@@ -602,16 +602,10 @@ hipError_t launch_step_impl(const Geo &g, const StepPtrs<T> &p, bool homog, cons
         if (zc == 2) return launch_colp<T, EXACT, 2>(g, p, homog, lc.stream);
         return launch_colp<T, EXACT, 1>(g, p, homog, lc.stream);
     }
-    if (lc.kernel == 0 && homog && col_fits && sizeof(T) == 4 && colx2_fits(g, homog, 4) &&
-        ((g.n[0] / 2) * g.n[1]) % 64 == 0) {
-        // binary32 homogeneous path: pairs with four z cells per thread where
-        // a plane's pairs fill whole waves (measured 10% faster at 32^3 x
-        // 16^3; with padded waves, as at 24^3, the column kernel is as fast)
-        return launch_colx2<T, EXACT, 4>(g, p, lc.stream);
-    }
-    if (lc.kernel == 0 && homog && col_fits && sizeof(T) == 8 && colx2_fits(g, homog, 1)) {
-        // binary64 homogeneous path: paired accesses (measured 10% faster at
-        // 24^3 x 16^3; no gain in binary32)
+    if (lc.kernel == 0 && homog && col_fits && colx2_fits(g, homog, 1)) {
+        // homogeneous path: paired accesses, two z cells per thread (binary64
+        // 24^3 x 16^3: 10% faster than the column kernel; binary32: 3% faster
+        // at 24^3 x 16^3, as fast as four z cells per thread at 32^3 x 16^3)
         if (g.n[2] % 2 == 0) return launch_colx2<T, EXACT, 2>(g, p, lc.stream);
         return launch_colx2<T, EXACT, 1>(g, p, lc.stream);
     }

@@ -165,17 +165,17 @@ def test_medium_packet_many_steps_against_oracle(exact, gpu_available):
 
 
 def test_fp32_default_kernel_choice_is_invisible(gpu_available):
-    """Where a plane's pairs fill whole waves, the binary32 default is the
-    pair kernel with four z cells per thread; it gives the column kernel's
-    values bit for bit, so ensembles (column body) and single contexts agree."""
+    """The binary32 homogeneous default is the pair kernel (two z cells per
+    thread); it gives the column kernel's and the four-z-cell pair kernel's
+    values bit for bit, so ensembles and single contexts agree."""
     case = cases.packet_case(n=(16, 8, 8, 4, 4, 4), name="fp32_pairs")   # 64 pairs per plane
     outs = {}
-    for kernel in (0, 5, 17):
+    for kernel in (0, 5, 16, 17):
         with _model(case, precision=rt6d.FP32, kernel=kernel) as m:
             m.dt = m.cfl_dt(0.5)
             m.integrate(9)
             outs[kernel] = m.download("E")
-    assert np.array_equal(outs[0], outs[5]) and np.array_equal(outs[0], outs[17])
+    assert all(np.array_equal(outs[0], outs[k]) for k in (5, 16, 17))
     with _model(case) as m:
         m.integrate(9, dt=m.cfl_dt(0.5))
         assert rel_l2(outs[0], m.download("E")) <= TOL32
