@@ -65,7 +65,8 @@ extern "C" {
 #define RT6D_OPT_BLOCK 3     /* threads per block of the per-cell step kernel (64..1024) */
 #define RT6D_OPT_KERNEL 4    /* step kernel: 0 auto (default: the column kernel with two z
                                 cells per thread when nz is even, with two x cells per
-                                thread on the homogeneous path when nx is even, with
+                                thread on the homogeneous path when nx is even (four in
+                                binary32 when nx is divisible by 4 and nz even), with
                                 the y neighbours staged in LDS on the general path);
                                 1 per-cell kernel;
                                 2/3 LDS marching kernel, 1/2 z cells per thread (nx*ny <= 1024);
@@ -78,7 +79,10 @@ extern "C" {
                                 y neighbour from LDS (nx*ny <= 1024);
                                 15/16/17 column kernel with two x cells per thread (paired
                                 accesses), 1/2/4 z cells per thread: homogeneous path with
-                                nx even, else the default */
+                                nx even, else the default;
+                                18/19/20 column kernel with four x cells per thread
+                                (16-byte accesses), 1/2/4 z cells per thread: binary32
+                                homogeneous path with nx divisible by 4, else the default */
 
 /* queries for rt6d_query */
 #define RT6D_Q_HOMOGENEOUS 1 /* 1 if the horizontally homogeneous path is active */

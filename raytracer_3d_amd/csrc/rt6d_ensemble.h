@@ -23,9 +23,11 @@ struct EnsMember {
     Geo g;
     StepPtrs<T> p[2];   // p[0]: E[a] -> E[b]; p[1]: E[b] -> E[a]
     ColCfg cc;
-    ColX2Cfg cx;        // pair-kernel configuration (variants with bit 3)
+    ColX2Cfg cx;        // pair- or quad-kernel configuration (variants with bit 3 or 4)
     uint32_t variant;   // bit 0: homogeneous tables; bit 1: two z cells per thread; bit 2: all axes
-                        // active; bit 3: two x cells per thread (homogeneous, nx even)
+                        // active; bit 3: two x cells per thread (homogeneous, nx even);
+                        // bit 4: four x cells per thread (binary32, homogeneous, nx
+                        // divisible by 4, two z cells per thread)
     uint32_t pad;
 };
 
@@ -33,7 +35,7 @@ struct EnsTask {
     uint32_t member, block;
 };
 
-constexpr int kEnsVariants = 16;
+constexpr int kEnsVariants = 32;
 constexpr int kEnsCounters = kEnsVariants * 8;   // one work counter per variant and XCD
 
 static __global__ void k_ens_reset(uint32_t *ctr)
@@ -41,14 +43,19 @@ static __global__ void k_ens_reset(uint32_t *ctr)
     if (threadIdx.x < kEnsCounters) ctr[threadIdx.x] = 0;
 }
 
-template <typename T, bool HOMOG, int ZC, bool FULL, bool X2>
+template <typename T, bool HOMOG, int ZC, bool FULL, bool X2, bool Q>
 __device__ __forceinline__ void ens_task(const EnsMember<T> &M, uint32_t parity, uint32_t block)
 {
-    if constexpr (X2) colx2_body<T, false, ZC, FULL>(M.g, M.p[parity & 1u], M.cx, block);
-    else col_body<T, false, HOMOG, ZC, FULL>(M.g, M.p[parity & 1u], M.cc, block);
+    if constexpr (Q) {
+        if constexpr (sizeof(T) == 4) colq_body<false, ZC, FULL>(M.g, M.p[parity & 1u], M.cx, block);
+    } else if constexpr (X2) {
+        colx2_body<T, false, ZC, FULL>(M.g, M.p[parity & 1u], M.cx, block);
+    } else {
+        col_body<T, false, HOMOG, ZC, FULL>(M.g, M.p[parity & 1u], M.cc, block);
+    }
 }
 
-template <typename T, bool HOMOG, int ZC, bool FULL, bool X2, bool PERSIST>
+template <typename T, bool HOMOG, int ZC, bool FULL, bool X2, bool Q, bool PERSIST>
 __global__ void __launch_bounds__(256) k_ens_step(const EnsMember<T> *__restrict__ mem,
                                                   const EnsTask *__restrict__ tasks, uint32_t ntasks,
                                                   uint32_t parity, uint32_t *ctr)
@@ -57,7 +64,7 @@ __global__ void __launch_bounds__(256) k_ens_step(const EnsMember<T> *__restrict
         const uint32_t t = xcd_logical_block(blockIdx.x, ntasks);
         if (t >= ntasks) return;   // uniform over the block
         const EnsTask tk = tasks[t];
-        ens_task<T, HOMOG, ZC, FULL, X2>(mem[tk.member], parity, tk.block);
+        ens_task<T, HOMOG, ZC, FULL, X2, Q>(mem[tk.member], parity, tk.block);
     } else {
         __shared__ uint32_t s_task;
         const uint32_t xcd = blockIdx.x & 7u;
@@ -71,7 +78,7 @@ __global__ void __launch_bounds__(256) k_ens_step(const EnsMember<T> *__restrict
             __syncthreads();   // every thread has read s_task before thread 0 writes it again
             if (t >= t1) break;
             const EnsTask tk = tasks[t];
-            ens_task<T, HOMOG, ZC, FULL, X2>(mem[tk.member], parity, tk.block);
+            ens_task<T, HOMOG, ZC, FULL, X2, Q>(mem[tk.member], parity, tk.block);
         }
     }
 }

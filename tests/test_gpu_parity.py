@@ -165,20 +165,56 @@ def test_medium_packet_many_steps_against_oracle(exact, gpu_available):
 
 
 def test_fp32_default_kernel_choice_is_invisible(gpu_available):
-    """The binary32 homogeneous default is the pair kernel (two z cells per
-    thread); it gives the column kernel's and the four-z-cell pair kernel's
-    values bit for bit, so ensembles and single contexts agree."""
+    """The binary32 homogeneous default is the quad kernel (four x and two z
+    cells per thread) where nx is divisible by 4 and nz even; it gives the
+    column kernel's and the pair kernels' values bit for bit, so ensembles
+    and single contexts agree."""
     case = cases.packet_case(n=(16, 8, 8, 4, 4, 4), name="fp32_pairs")   # 64 pairs per plane
     outs = {}
-    for kernel in (0, 5, 16, 17):
+    for kernel in (0, 5, 16, 17, 19):
         with _model(case, precision=rt6d.FP32, kernel=kernel) as m:
             m.dt = m.cfl_dt(0.5)
             m.integrate(9)
             outs[kernel] = m.download("E")
-    assert all(np.array_equal(outs[0], outs[k]) for k in (5, 16, 17))
+    assert all(np.array_equal(outs[0], outs[k]) for k in (5, 16, 17, 19))
     with _model(case) as m:
         m.integrate(9, dt=m.cfl_dt(0.5))
         assert rel_l2(outs[0], m.download("E")) <= TOL32
+
+
+@pytest.mark.parametrize("n", [(16, 8, 8, 4, 4, 4), (12, 5, 6, 4, 3, 5), (8, 1, 4, 4, 4, 4), (4, 6, 4, 3, 4, 4),
+                               (24, 24, 4, 4, 4, 4)], ids=lambda n: "x".join(map(str, n)))
+def test_fp32_quad_kernel_is_bitwise_the_column_kernel(n, gpu_available):
+    """The quad kernel (four x cells per thread, options 18/19/20) gives the
+    column kernel's binary32 values bit for bit: one quad per row (nx = 4),
+    ragged planes (padding lanes), inactive y, and whole-wave planes."""
+    case = cases.packet_case(n=n, name="fp32_quads")
+    outs = {}
+    for kernel in (5, 18, 19, 20):
+        with _model(case, precision=rt6d.FP32, kernel=kernel) as m:
+            assert m.homogeneous
+            m.dt = m.cfl_dt(0.5)
+            m.integrate(9)
+            outs[kernel] = m.download("E")
+    for kernel in (18, 19, 20):
+        assert np.array_equal(outs[kernel], outs[5]), kernel
+    with _model(case) as m:
+        m.integrate(9, dt=m.cfl_dt(0.5))
+        assert rel_l2(outs[19], m.download("E")) <= TOL32
+
+
+@pytest.mark.parametrize("kernel", [18, 19, 20], ids=["colq1", "colq2", "colq4"])
+@pytest.mark.parametrize("entry", ENTRIES, ids=IDS)
+def test_fp32_quad_kernel_against_reference(entry, kernel, gpu_available):
+    case = golden_case(entry)
+    g = golden_data(entry)
+    outs = []
+    for k in (kernel, 0):
+        with _model(case, precision=rt6d.FP32, kernel=k) as m:
+            m.integrate(int(g["nsteps"]))
+            outs.append(m.download("E"))
+    assert np.array_equal(outs[0], outs[1])
+    assert rel_l2(outs[0], g["EN"]) <= TOL32
 
 
 # ----------------------------------------------------------- production size
