@@ -66,7 +66,8 @@ extern "C" {
 #define RT6D_OPT_KERNEL 4    /* step kernel: 0 auto (default: the column kernel with two z
                                 cells per thread when nz is even, with two x cells per
                                 thread on the homogeneous path when nx is even (four in
-                                binary32 when nx is divisible by 4 and nz even), with
+                                binary32 when nx is divisible by 4, nz even and the grid
+                                gives each SIMD 2.5 wavefronts of quads), with
                                 the y neighbours staged in LDS on the general path);
                                 1 per-cell kernel;
                                 2/3 LDS marching kernel, 1/2 z cells per thread (nx*ny <= 1024);

@@ -61,6 +61,7 @@ struct rt6d_ctx {
     Geo g{};
     uint64_t cells = 0;
     int device = 0;
+    int ncu = 256;   // compute units of the device (kernel choice)
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double f0 = 1e-4, sign = 1.0;
@@ -429,6 +430,11 @@ int rt6d_create(const int *n, int reflect_up, int reflect_down, int precision, r
     c->g.up = c->up;
     c->g.down = c->down;
     hipGetDevice(&c->device);
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
+            c->ncu = prop.multiProcessorCount;
+    }
     int r = 0;
     auto bail = [&](int code) {
         rt6d_destroy(c);
@@ -783,7 +789,7 @@ static int do_steps(rt6d_ctx *c, double dt, int nsteps)
     if (nsteps < 0 || !std::isfinite(dt)) return fail(RT6D_EINVAL, "rt6d_step: bad arguments");
     if (nsteps == 0) return 0;
     if ((r = ensure_coef(c, dt))) return r;
-    const LaunchCfg lc{c->opt_block, c->opt_kernel, c->stream};
+    const LaunchCfg lc{c->opt_block, c->opt_kernel, c->ncu, c->stream};
     const double dz1 = c->dt_h[2][2];   // dzt(1): halo slot f+1 holds Fortran index f
     for (int s = 0; s < nsteps; ++s) {
         void *in = c->d_E[c->cur], *out = c->d_E[c->cur ^ 1];

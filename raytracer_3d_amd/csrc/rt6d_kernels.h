@@ -402,6 +402,7 @@ __global__ void __launch_bounds__(256) k_diag_columns(const Geo g, const StepPtr
 struct LaunchCfg {
     int block;          // threads per block of the per-cell kernel
     int kernel;         // 0 auto, 1 per-cell kernel, 2 marching kernel
+    int ncu;            // compute units of the device
     hipStream_t stream;
 };
 
@@ -564,6 +565,18 @@ inline bool colq_fits(const Geo &g, bool homog, int zc)
     return homog && g.n[0] >= 4 && g.n[0] % 4 == 0 && g.n[2] % zc == 0;
 }
 
+// The quad kernel has half the pair kernel's threads.  On grids too small
+// to give each SIMD (4 per compute unit) 2.5 wavefronts of quads it leaves
+// the device underfilled and the pair kernel is faster (binary32, measured
+// over the serving member sizes: 12^3 x 8^3 0.022 against 0.017 ms, 16^3 x
+// 16^3 0.068 against 0.063 ms; 20^3 x 12^3 0.056 against 0.064 ms).
+inline bool colq_fills(const Geo &g, int zc, int ncu)
+{
+    const uint64_t waves_per_plane = ((uint64_t)(g.n[0] / 4) * g.n[1] + 63u) / 64u;
+    const uint64_t waves = waves_per_plane * (uint64_t)(g.n[2] / zc) * g.n[3] * g.n[4];
+    return 2 * waves >= 5ull * 4u * (uint64_t)(ncu > 0 ? ncu : 256);
+}
+
 template <bool EXACT, int ZC>
 hipError_t launch_colq(const Geo &g, const StepPtrs<float> &p, hipStream_t stream)
 {
@@ -641,7 +654,7 @@ hipError_t launch_step_impl(const Geo &g, const StepPtrs<T> &p, bool homog, cons
         return launch_colp<T, EXACT, 1>(g, p, homog, lc.stream);
     }
     if constexpr (sizeof(T) == 4) {
-        if (lc.kernel == 0 && homog && col_fits && colq_fits(g, homog, 2)) {
+        if (lc.kernel == 0 && homog && col_fits && colq_fits(g, homog, 2) && colq_fills(g, 2, lc.ncu)) {
             // binary32 homogeneous path: quads, two z cells per thread
             // (24^3 x 16^3: 0.221 against the pair kernel's 0.231 ms;
             // 32^3 x 16^3: 0.430 against 0.497 ms)

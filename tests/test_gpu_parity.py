@@ -166,9 +166,9 @@ def test_medium_packet_many_steps_against_oracle(exact, gpu_available):
 
 def test_fp32_default_kernel_choice_is_invisible(gpu_available):
     """The binary32 homogeneous default is the quad kernel (four x and two z
-    cells per thread) where nx is divisible by 4 and nz even; it gives the
-    column kernel's and the pair kernels' values bit for bit, so ensembles
-    and single contexts agree."""
+    cells per thread) where nx is divisible by 4, nz even and the grid fills
+    the device, else the pair kernel; all of them give the column kernel's
+    values bit for bit, so ensembles and single contexts agree."""
     case = cases.packet_case(n=(16, 8, 8, 4, 4, 4), name="fp32_pairs")   # 64 pairs per plane
     outs = {}
     for kernel in (0, 5, 16, 17, 19):
