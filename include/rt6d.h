@@ -65,9 +65,11 @@ extern "C" {
 #define RT6D_OPT_BLOCK 3     /* threads per block of the per-cell step kernel (64..1024) */
 #define RT6D_OPT_KERNEL 4    /* step kernel: 0 auto (default: the column kernel with two z
                                 cells per thread when nz is even, with two x cells per
-                                thread on the homogeneous path when nx is even (four in
-                                binary32 when nx is divisible by 4, nz even and the grid
-                                gives each SIMD 2.5 wavefronts of quads), with
+                                thread on the homogeneous path when nx is even; in
+                                binary32, where the grid gives each SIMD 2.5 wavefronts,
+                                with the wave-local neighbour exchange (option 22) when
+                                nx <= 24, else four x cells per thread (option 19) when
+                                nx is divisible by 4 and nz even; with
                                 the y neighbours staged in LDS on the general path);
                                 1 per-cell kernel;
                                 2/3 LDS marching kernel, 1/2 z cells per thread (nx*ny <= 1024);
@@ -83,7 +85,13 @@ extern "C" {
                                 nx even, else the default;
                                 18/19/20 column kernel with four x cells per thread
                                 (16-byte accesses), 1/2/4 z cells per thread: binary32
-                                homogeneous path with nx divisible by 4, else the default */
+                                homogeneous path with nx divisible by 4, else the default;
+                                21/22/23 the pair kernel with the x and y neighbour pairs
+                                exchanged through wave-local LDS slots: homogeneous path,
+                                nx even and <= 32, else the default;
+                                24/25/26 the quad kernel with the same exchange: binary32
+                                homogeneous path, nx divisible by 4 and <= 64, else the
+                                default */
 
 /* queries for rt6d_query */
 #define RT6D_Q_HOMOGENEOUS 1 /* 1 if the horizontally homogeneous path is active */

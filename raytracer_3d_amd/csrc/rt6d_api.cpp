@@ -1063,7 +1063,7 @@ int rt6d_set_option(rt6d_ctx *c, int option, int value)
         c->opt_block = value;
         return 0;
     case RT6D_OPT_KERNEL:
-        if (value < 0 || value > 20) return fail(RT6D_EINVAL, "kernel option %d", value);
+        if (value < 0 || value > 26) return fail(RT6D_EINVAL, "kernel option %d", value);
         if ((value == 16 && c->n[2] % 2 != 0) || (value == 17 && c->n[2] % 4 != 0))
             return fail(RT6D_EINVAL, "pair kernel: nz (%d) not divisible by its z cells per thread", c->n[2]);
         if (value >= 12 && value <= 14) {
@@ -1178,14 +1178,18 @@ int ens_rebuild_t(rt6d_ensemble *e)
             M.p[0] = step_ptrs<T>(c, c->d_E[a], c->d_E[a ^ 1], e->dt[i]);
             M.p[1] = step_ptrs<T>(c, c->d_E[a ^ 1], c->d_E[a], e->dt[i]);
             const bool full = c->homog ? is_full<true>(c->g) : is_full<false>(c->g);
-            // the single-context kernel choice: quads in binary32 where they
-            // fit with two z cells per thread, else pairs where nx is even
-            const bool q4 = sizeof(T) == 4 && zc == 2 && colq_fits(c->g, c->homog, 2);
-            const bool x2 = !q4 && colx2_fits(c->g, c->homog, zc);
-            v = (c->homog ? 1 : 0) | (zc == 2 ? 2 : 0) | (full ? 4 : 0) | (x2 ? 8 : 0) | (q4 ? 16 : 0);
+            // the single-context kernel choice on a grid that fills the
+            // device (the members together do): in binary32 with two z cells
+            // per thread the wave-local pair kernel where nx <= 24, else
+            // quads where they fit; else pairs where nx is even
+            const bool wl = sizeof(T) == 4 && zc == 2 && c->g.n[0] <= 24 && colx2wl_fits(c->g, c->homog, 2);
+            const bool q4 = !wl && sizeof(T) == 4 && zc == 2 && colq_fits(c->g, c->homog, 2);
+            const bool x2 = !wl && !q4 && colx2_fits(c->g, c->homog, zc);
+            v = (c->homog ? 1 : 0) | (zc == 2 ? 2 : 0) | (full ? 4 : 0) | (x2 ? 8 : 0) | (q4 ? 16 : 0) |
+                (wl ? 32 : 0);
             M.variant = (uint32_t)v;
             e->cur0[i] = a;
-            if (x2 || q4) {
+            if (x2 || q4 || wl) {
                 const uint32_t w = q4 ? 4u : 2u;   // x cells per thread
                 const uint32_t ngroups = (uint32_t)(c->g.n[0] / w) * c->g.n[1];
                 M.cx.nxyp = (int)(((ngroups + 63u) / 64u) * 64u);

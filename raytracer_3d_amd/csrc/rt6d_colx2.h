@@ -74,7 +74,17 @@ struct ColX2Cfg {
     int nzb;       // nz / ZC
 };
 
-template <typename T, bool EXACT, int ZC, bool FULL>
+// Wave-local neighbour exchange (WL): each wavefront of the block keeps, per
+// z cell, 128 pair slots in LDS: H = nx pairs (two rows) before its 64 pairs,
+// its own pairs, and H after (nx <= 32).  Every step a wave writes its own
+// pairs and loads the halo pairs once (one 16-byte access by lanes 0..2H-1),
+// then reads the x and y neighbour pairs from the slots.  Only the wave's own
+// lanes touch its slots, and the LDS executes a wave's accesses in order, so
+// no barrier is needed; wave_barrier keeps the compiler from moving the
+// reads above the writes.
+constexpr int kWLSlots = 128;
+
+template <typename T, bool EXACT, int ZC, bool FULL, bool WL = false>
 __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, const ColX2Cfg &cc, uint32_t b)
 {
     const uint32_t t = b * blockDim.x + threadIdx.x;
@@ -87,9 +97,16 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
     const int bl = (int)fdiv(kl, g.div[3]);
     const int bk = (int)kl - bl * nk;
     if (bl >= nl) return;          // whole wavefronts beyond the grid
-    if (xyp >= nxh * ny) return;   // padding lanes (no barriers below)
-    const int y0 = (int)fdiv((uint32_t)xyp, cc.dxh);
-    const int x0 = 2 * (xyp - y0 * nxh);   // 0-based first cell of the pair
+    const int npairs = nxh * ny;
+    if constexpr (!WL) {
+        if (xyp >= npairs) return;   // padding lanes (no barriers below)
+    }
+    // WL: padding lanes stay to load halo pairs; they compute on the plane's
+    // last pair and store nothing
+    const bool live = xyp < npairs;
+    const int xyc = live ? xyp : npairs - 1;
+    const int y0 = (int)fdiv((uint32_t)xyc, cc.dxh);
+    const int x0 = 2 * (xyc - y0 * nxh);   // 0-based first cell of the pair
     const int x = x0 + 1, y = y0 + 1;      // 1-based: the pair is (x, x+1)
     const int z0 = zb * ZC + 1, k = bk + 1, l = bl + 1;
     constexpr uint32_t S = sizeof(T);
@@ -144,6 +161,29 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
             zmir[q] = mir;
         }
     }
+    // WL slots: the wave's first pair, the halo pair this lane loads, and the
+    // slots of the x and y neighbour pairs
+    using V2 = T __attribute__((ext_vector_type(2)));
+    const int H = nx;                              // pairs in two rows
+    const int lane = (int)(threadIdx.x & 63u), wv = (int)(threadIdx.x >> 6);
+    const int pw = xyp & ~63;
+    int hslot = 0, hp = -1;
+    if (lane < H) { hp = pw - H + lane; hslot = lane; }
+    else if (lane < 2 * H) { hp = pw + 64 + (lane - H); hslot = 64 + lane; }
+    const bool hval = WL && hp >= 0 && hp < npairs;
+    uint32_t ohal = 0;
+    if (hval) {
+        const int yh = hp / nxh;
+        ohal = (uint32_t)(2 * (hp - yh * nxh) + nx * yh) * S;
+    }
+    int sy[4] = {0, 0, 0, 0};
+    if constexpr (WL) {
+        const int ys[4] = {y - 2 < 1 ? 1 : y - 2, y - 1 < 1 ? 1 : y - 1, y + 1 > ny ? ny : y + 1,
+                           y + 2 > ny ? ny : y + 2};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sy[q] = (ys[q] - 1) * nxh + (x0 >> 1) - pw + H;
+    }
+    const int sown = xyc - pw + H;
     auto poff = [&](int j, int mm) -> uint32_t { return base0 + (uint32_t)j * planeB + (uint32_t)(mm - 1) * mB; };
     auto ldm = [&](int j, int mm) -> Pair<T> {
         mm = mm < 1 ? 1 : (mm > nm ? nm : mm);
@@ -183,6 +223,14 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) zq[q] = az ? ldz(q, m) : Pair<T>{T(0), T(0)};
+        Pair<T> hq[ZC];
+        if constexpr (WL) {
+#pragma unroll
+            for (int j = 0; j < ZC; ++j) {
+                hq[j] = Pair<T>{T(0), T(0)};
+                if (hval) hq[j] = bld2<T>(rE, ohal, (P0 + (uint32_t)j + (uint32_t)(m - 1) * nzkl) * planeB);
+            }
+        }
 
         T FzA[ZC + 1], FzB[ZC + 1];
         if (az) {
@@ -211,8 +259,36 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
             const uint32_t q2 = q20 + (uint32_t)j;
             const Pair<T> e = w0[j];
             T dEa = T(0), dEb = T(0);
+            [[maybe_unused]] V2 *sl = nullptr;
+            if constexpr (WL) {
+                __shared__ V2 sW[4][ZC][kWLSlots];
+                sl = &sW[wv][j][0];
+                __builtin_amdgcn_wave_barrier();   // the previous step's reads stay above
+                V2 own;
+                own.x = e.a;
+                own.y = e.b;
+                sl[sown] = own;
+                if (hval) {
+                    V2 h;
+                    h.x = hq[j].a;
+                    h.y = hq[j].b;
+                    sl[hslot] = h;
+                }
+                __builtin_amdgcn_wave_barrier();   // the reads below stay below
+            }
+            auto slot = [&](int i) -> Pair<T> {
+                const V2 v = sl[i];
+                return Pair<T>{v.x, v.y};
+            };
             if (ax) {
-                Pair<T> L = bld2<T>(rE, oxl, pb), R = bld2<T>(rE, oxr, pb);
+                Pair<T> L, R;
+                if constexpr (WL) {
+                    L = slot(sown - 1);
+                    R = slot(sown + 1);
+                } else {
+                    L = bld2<T>(rE, oxl, pb);
+                    R = bld2<T>(rE, oxr, pb);
+                }
                 if (xfirst) L = Pair<T>{e.a, e.a};
                 if (xlast) R = Pair<T>{e.b, e.b};
                 const T v0 = sld(p.hom.vx3, q2), vi = sld(p.hom.vx3, p.hom.n4 + q2), vb = sld(p.hom.vx3, 2 * p.hom.n4 + q2);
@@ -224,8 +300,18 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
                 dEb = div_term<T, EXACT>(dEb, F2, F1, dx2, rxb);
             }
             if (ay) {
-                const Pair<T> a0 = bld2<T>(rE, oy[0], pb), a1 = bld2<T>(rE, oy[1], pb);
-                const Pair<T> a3 = bld2<T>(rE, oy[2], pb), a4 = bld2<T>(rE, oy[3], pb);
+                Pair<T> a0, a1, a3, a4;
+                if constexpr (WL) {
+                    a0 = slot(sy[0]);
+                    a1 = slot(sy[1]);
+                    a3 = slot(sy[2]);
+                    a4 = slot(sy[3]);
+                } else {
+                    a0 = bld2<T>(rE, oy[0], pb);
+                    a1 = bld2<T>(rE, oy[1], pb);
+                    a3 = bld2<T>(rE, oy[2], pb);
+                    a4 = bld2<T>(rE, oy[3], pb);
+                }
                 __builtin_amdgcn_s_setprio(0);
                 const T v0 = sld(p.hom.vy3, q2), vi = sld(p.hom.vy3, p.hom.n4 + q2), vb = sld(p.hom.vy3, 2 * p.hom.n4 + q2);
                 const T vL = y == 1 ? v0 : vi, vR = y == ny ? vb : vi;
@@ -263,7 +349,7 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
                 en.a = e.a + (T)p.dt * dEa;
                 en.b = e.b + (T)p.dt * dEb;
             }
-            bst2<T>(en, rO, oc, pb);
+            if (!WL || live) bst2<T>(en, rO, oc, pb);
         }
 #pragma unroll
         for (int j = 0; j < ZC; ++j) {
@@ -275,10 +361,10 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
     }
 }
 
-template <typename T, bool EXACT, int ZC, bool FULL>
+template <typename T, bool EXACT, int ZC, bool FULL, bool WL = false>
 __global__ void __launch_bounds__(256) k_step_colx2(const Geo g, const StepPtrs<T> p, const ColX2Cfg cc, uint32_t nbl)
 {
     const uint32_t b = xcd_logical_block(blockIdx.x, nbl);
     if (b >= nbl) return;
-    colx2_body<T, EXACT, ZC, FULL>(g, p, cc, b);
+    colx2_body<T, EXACT, ZC, FULL, WL>(g, p, cc, b);
 }

@@ -538,7 +538,22 @@ inline bool colx2_fits(const Geo &g, bool homog, int zc)
     return homog && g.n[0] >= 2 && g.n[0] % 2 == 0 && g.n[2] % zc == 0;
 }
 
-template <typename T, bool EXACT, int ZC>
+// The wave-local LDS variant needs the halo of two rows on each side of a
+// wavefront's 64 pairs in one access: nx <= 32.
+inline bool colx2wl_fits(const Geo &g, bool homog, int zc) { return colx2_fits(g, homog, zc) && g.n[0] <= 32; }
+
+// As for the quad kernel: a grid that gives each SIMD 2.5 wavefronts of
+// pairs.  Below that the wave-local variant's halo loads and LDS exchange
+// cost more than they save (binary32: 12^3 x 8^3 0.0182 against 0.0170 ms,
+// 16^3 x 12^3 0.0354 against 0.0371, 24^3 x 8^3 0.0292 against 0.0305).
+inline bool colx2_fills(const Geo &g, int zc, int ncu)
+{
+    const uint64_t waves_per_plane = ((uint64_t)(g.n[0] / 2) * g.n[1] + 63u) / 64u;
+    const uint64_t waves = waves_per_plane * (uint64_t)(g.n[2] / zc) * g.n[3] * g.n[4];
+    return 2 * waves >= 5ull * 4u * (uint64_t)(ncu > 0 ? ncu : 256);
+}
+
+template <typename T, bool EXACT, int ZC, bool WL = false>
 hipError_t launch_colx2(const Geo &g, const StepPtrs<T> &p, hipStream_t stream)
 {
     const uint32_t npairs = (uint32_t)(g.n[0] / 2) * g.n[1];
@@ -552,9 +567,9 @@ hipError_t launch_colx2(const Geo &g, const StepPtrs<T> &p, hipStream_t stream)
     const uint32_t nbl = (uint32_t)((nthreads + 255) / 256);
     const uint32_t nbh = ((nbl + 7u) / 8u) * 8u;
     if (is_full<true>(g))
-        hipLaunchKernelGGL((k_step_colx2<T, EXACT, ZC, true>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
+        hipLaunchKernelGGL((k_step_colx2<T, EXACT, ZC, true, WL>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
     else
-        hipLaunchKernelGGL((k_step_colx2<T, EXACT, ZC, false>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
+        hipLaunchKernelGGL((k_step_colx2<T, EXACT, ZC, false, WL>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
     return hipGetLastError();
 }
 
@@ -577,7 +592,10 @@ inline bool colq_fills(const Geo &g, int zc, int ncu)
     return 2 * waves >= 5ull * 4u * (uint64_t)(ncu > 0 ? ncu : 256);
 }
 
-template <bool EXACT, int ZC>
+// The wave-local LDS variant of the quad kernel: nx <= 64.
+inline bool colqwl_fits(const Geo &g, bool homog, int zc) { return colq_fits(g, homog, zc) && g.n[0] <= 64; }
+
+template <bool EXACT, int ZC, bool WL = false>
 hipError_t launch_colq(const Geo &g, const StepPtrs<float> &p, hipStream_t stream)
 {
     const uint32_t nquads = (uint32_t)(g.n[0] / 4) * g.n[1];
@@ -591,9 +609,9 @@ hipError_t launch_colq(const Geo &g, const StepPtrs<float> &p, hipStream_t strea
     const uint32_t nbl = (uint32_t)((nthreads + 255) / 256);
     const uint32_t nbh = ((nbl + 7u) / 8u) * 8u;
     if (is_full<true>(g))
-        hipLaunchKernelGGL((k_step_colq<EXACT, ZC, true>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
+        hipLaunchKernelGGL((k_step_colq<EXACT, ZC, true, WL>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
     else
-        hipLaunchKernelGGL((k_step_colq<EXACT, ZC, false>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
+        hipLaunchKernelGGL((k_step_colq<EXACT, ZC, false, WL>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
     return hipGetLastError();
 }
 
@@ -637,7 +655,23 @@ hipError_t launch_step_impl(const Geo &g, const StepPtrs<T> &p, bool homog, cons
             return launch_colx2<T, EXACT, 1>(g, p, lc.stream);
         }
     }
+    if (lc.kernel >= 21 && lc.kernel <= 23 && col_fits) {
+        const int zc = lc.kernel == 21 ? 1 : (lc.kernel == 22 ? 2 : 4);
+        if (colx2wl_fits(g, homog, zc)) {
+            if (zc == 4) return launch_colx2<T, EXACT, 4, true>(g, p, lc.stream);
+            if (zc == 2) return launch_colx2<T, EXACT, 2, true>(g, p, lc.stream);
+            return launch_colx2<T, EXACT, 1, true>(g, p, lc.stream);
+        }
+    }
     if constexpr (sizeof(T) == 4) {
+        if (lc.kernel >= 24 && lc.kernel <= 26 && col_fits) {
+            const int zc = lc.kernel == 24 ? 1 : (lc.kernel == 25 ? 2 : 4);
+            if (colqwl_fits(g, homog, zc)) {
+                if (zc == 4) return launch_colq<EXACT, 4, true>(g, p, lc.stream);
+                if (zc == 2) return launch_colq<EXACT, 2, true>(g, p, lc.stream);
+                return launch_colq<EXACT, 1, true>(g, p, lc.stream);
+            }
+        }
         if (lc.kernel >= 18 && lc.kernel <= 20 && col_fits) {
             const int zc = lc.kernel == 18 ? 1 : (lc.kernel == 19 ? 2 : 4);
             if (colq_fits(g, homog, zc)) {
@@ -654,10 +688,17 @@ hipError_t launch_step_impl(const Geo &g, const StepPtrs<T> &p, bool homog, cons
         return launch_colp<T, EXACT, 1>(g, p, homog, lc.stream);
     }
     if constexpr (sizeof(T) == 4) {
+        if (lc.kernel == 0 && homog && col_fits && g.n[0] <= 24 && colx2wl_fits(g, homog, 2) &&
+            colx2_fills(g, 2, lc.ncu)) {
+            // binary32 homogeneous path, nx <= 24: pairs with the wave-local
+            // neighbour exchange (24^3 x 16^3: 0.204 against the quad kernel's
+            // 0.222 and the pair kernel's 0.231 ms)
+            return launch_colx2<T, EXACT, 2, true>(g, p, lc.stream);
+        }
         if (lc.kernel == 0 && homog && col_fits && colq_fits(g, homog, 2) && colq_fills(g, 2, lc.ncu)) {
-            // binary32 homogeneous path: quads, two z cells per thread
-            // (24^3 x 16^3: 0.221 against the pair kernel's 0.231 ms;
-            // 32^3 x 16^3: 0.430 against 0.497 ms)
+            // binary32 homogeneous path, wider rows: quads, two z cells per
+            // thread (32^3 x 16^3: 0.430 against the pair kernel's 0.497 and
+            // the wave-local pair kernel's 0.468 ms)
             return launch_colq<EXACT, 2>(g, p, lc.stream);
         }
     }

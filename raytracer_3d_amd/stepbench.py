@@ -32,7 +32,7 @@ def main(argv=None):
     ap.add_argument("--path", default="auto", choices=["auto", "general"])
     ap.add_argument("--kernel", default="auto", choices=["auto", "percell", "march1", "march2", "col1", "col2", "col4", "plane1", "plane2",
                                                         "coly1", "coly2", "coly4", "colp1", "colp2", "colp4",
-                                                        "colx1", "colx2", "colx4", "colq1", "colq2", "colq4"])
+                                                        "colx1", "colx2", "colx4", "colq1", "colq2", "colq4", "colw1", "colw2", "colw4", "colqw1", "colqw2", "colqw4"])
     ap.add_argument("--exact", action="store_true")
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -46,7 +46,8 @@ def main(argv=None):
     kern = {"auto": 0, "percell": 1, "march1": 2, "march2": 3, "col1": 4, "col2": 5, "col4": 6, "plane1": 7,
             "plane2": 8, "coly1": 9, "coly2": 10, "coly4": 11,
             "colp1": 12, "colp2": 13, "colp4": 14, "colx1": 15, "colx2": 16, "colx4": 17,
-            "colq1": 18, "colq2": 19, "colq4": 20}[a.kernel]
+            "colq1": 18, "colq2": 19, "colq4": 20, "colw1": 21, "colw2": 22, "colw4": 23,
+            "colqw1": 24, "colqw2": 25, "colqw4": 26}[a.kernel]
     m = rt6d.Model.from_case(case, precision=prec, path=1 if a.path == "general" else 0, kernel=kern,
                              exact=a.exact)
     m.calc_parameter()

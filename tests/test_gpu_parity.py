@@ -44,18 +44,19 @@ def test_calc_parameter_bitwise(entry, gpu_available):
         assert m.homogeneous == (entry["stem"] in HOMOG_EXPECTED)
 
 
-@pytest.mark.parametrize("kernel", [0, 1, 2, 3, 4, 5, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17],
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3, 4, 5, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 21, 22, 23],
                          ids=["auto", "percell", "march1", "march2", "col1", "col2", "plane1", "plane2",
-                              "coly1", "coly2", "coly4", "colp1", "colp2", "colp4", "colx1", "colx2", "colx4"])
+                              "coly1", "coly2", "coly4", "colp1", "colp2", "colp4", "colx1", "colx2", "colx4",
+                              "colw1", "colw2", "colw4"])
 @pytest.mark.parametrize("exact", [True, False], ids=["exact", "fast"])
 @pytest.mark.parametrize("path", [0, 1], ids=["auto", "general"])
 @pytest.mark.parametrize("entry", ENTRIES, ids=IDS)
 def test_integrate_matches_reference(entry, path, exact, kernel, gpu_available):
     case = golden_case(entry)
     g = golden_data(entry)
-    if kernel in (3, 5, 8, 10, 13, 16) and case.n[2] % 2:
+    if kernel in (3, 5, 8, 10, 13, 16, 22) and case.n[2] % 2:
         pytest.skip("two z cells per thread need an even nz")
-    if kernel in (11, 14, 17) and case.n[2] % 4:
+    if kernel in (11, 14, 17, 23) and case.n[2] % 4:
         pytest.skip("four z cells per thread need nz divisible by 4")
     with _model(case, exact=exact, path=path, kernel=kernel) as m:
         if path == 1:
@@ -171,12 +172,12 @@ def test_fp32_default_kernel_choice_is_invisible(gpu_available):
     values bit for bit, so ensembles and single contexts agree."""
     case = cases.packet_case(n=(16, 8, 8, 4, 4, 4), name="fp32_pairs")   # 64 pairs per plane
     outs = {}
-    for kernel in (0, 5, 16, 17, 19):
+    for kernel in (0, 5, 16, 17, 19, 21, 22, 23, 25):
         with _model(case, precision=rt6d.FP32, kernel=kernel) as m:
             m.dt = m.cfl_dt(0.5)
             m.integrate(9)
             outs[kernel] = m.download("E")
-    assert all(np.array_equal(outs[0], outs[k]) for k in (5, 16, 17, 19))
+    assert all(np.array_equal(outs[0], outs[k]) for k in (5, 16, 17, 19, 21, 22, 23, 25))
     with _model(case) as m:
         m.integrate(9, dt=m.cfl_dt(0.5))
         assert rel_l2(outs[0], m.download("E")) <= TOL32
@@ -201,6 +202,28 @@ def test_fp32_quad_kernel_is_bitwise_the_column_kernel(n, gpu_available):
     with _model(case) as m:
         m.integrate(9, dt=m.cfl_dt(0.5))
         assert rel_l2(outs[19], m.download("E")) <= TOL32
+
+
+@pytest.mark.parametrize("precision", [rt6d.FP64, rt6d.FP32], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("n", [(24, 24, 4, 4, 4, 4), (32, 8, 4, 4, 3, 4), (12, 5, 6, 4, 3, 5), (2, 7, 4, 3, 4, 4),
+                               (30, 3, 4, 4, 4, 4), (16, 1, 4, 4, 4, 4), (4, 9, 4, 3, 3, 4), (64, 3, 4, 3, 3, 4)],
+                         ids=lambda n: "x".join(map(str, n)))
+def test_wave_local_pair_kernel_is_bitwise_the_column_kernel(n, precision, gpu_available):
+    """The pair kernel with x and y neighbours exchanged through wave-local
+    LDS slots (options 21/22/23), and in binary32 the quad kernel with the
+    same exchange (24/25/26), give the column kernel's values bit for bit:
+    halos cut by the plane's ends, ragged last waves (padding lanes that load
+    halo pairs), a single pair or quad per row, inactive y and nx up to 32."""
+    case = cases.packet_case(n=n, name="wave_local")
+    outs = {}
+    for kernel in (5, 21, 22, 23, 24, 25, 26):
+        with _model(case, precision=precision, kernel=kernel) as m:
+            assert m.homogeneous
+            m.dt = m.cfl_dt(0.5)
+            m.integrate(9)
+            outs[kernel] = m.download("E")
+    for kernel in (21, 22, 23, 24, 25, 26):
+        assert np.array_equal(outs[kernel], outs[5]), kernel
 
 
 @pytest.mark.parametrize("kernel", [18, 19, 20], ids=["colq1", "colq2", "colq4"])
