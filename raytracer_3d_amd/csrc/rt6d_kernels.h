@@ -543,9 +543,11 @@ inline bool colx2_fits(const Geo &g, bool homog, int zc)
 inline bool colx2wl_fits(const Geo &g, bool homog, int zc) { return colx2_fits(g, homog, zc) && g.n[0] <= 32; }
 
 // As for the quad kernel: a grid that gives each SIMD 2.5 wavefronts of
-// pairs.  Below that the wave-local variant's halo loads and LDS exchange
-// cost more than they save (binary32: 12^3 x 8^3 0.0182 against 0.0170 ms,
-// 16^3 x 12^3 0.0354 against 0.0371, 24^3 x 8^3 0.0292 against 0.0305).
+// pairs.  On smaller grids the wave-local variant's halo loads and LDS
+// exchange mostly cost more than they save (binary32, wave-local against
+// plain pairs: 12^3 x 8^3 0.0182 against 0.0170 ms, 20^3 x 8^3 0.0236
+// against 0.0225; 16^3 x 12^3, just below the threshold, 0.0354 against
+// 0.0371; 24^3 x 8^3, above it, 0.0292 against 0.0305).
 inline bool colx2_fills(const Geo &g, int zc, int ncu)
 {
     const uint64_t waves_per_plane = ((uint64_t)(g.n[0] / 2) * g.n[1] + 63u) / 64u;
