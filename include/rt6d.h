@@ -99,6 +99,10 @@ extern "C" {
 #define RT6D_Q_CELLS 3
 #define RT6D_Q_STEPS 4       /* steps taken since the last upload of E */
 #define RT6D_Q_BYTES_PER_CELL 5 /* algorithmic HBM bytes per cell-update of the active step kernel */
+#define RT6D_Q_KERNEL 6      /* the RT6D_OPT_KERNEL option rt6d_step runs: the set option, or
+                                what the auto choice resolves to (after calc_parameter, which
+                                decides the homogeneous path); an option that does not fit
+                                the grid runs the column kernel */
 
 typedef struct rt6d_ctx rt6d_ctx;
 

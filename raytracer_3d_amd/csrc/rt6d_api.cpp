@@ -1101,6 +1101,11 @@ int rt6d_query(rt6d_ctx *c, int what, long long *value)
     case RT6D_Q_DEVICE_BYTES: *value = (long long)c->dev_bytes; return 0;
     case RT6D_Q_CELLS: *value = (long long)c->cells; return 0;
     case RT6D_Q_STEPS: *value = c->steps; return 0;
+    case RT6D_Q_KERNEL:
+        *value = c->opt_kernel ? c->opt_kernel
+                               : (c->prec == RT6D_FP64 ? auto_kernel<double>(c->g, c->homog, c->ncu)
+                                                       : auto_kernel<float>(c->g, c->homog, c->ncu));
+        return 0;
     case RT6D_Q_BYTES_PER_CELL: {
         // E read + E write, plus the streamed coefficient fields on the general path
         double b = 2.0 * c->esz;

@@ -629,9 +629,38 @@ inline int col_zc(const Geo &g, int kernel)
 // One step.  kernel option: 0 auto (column kernel), 1 per-cell kernel,
 // 2/3 LDS marching kernel with 1/2 z cells per thread (nx*ny <= 1024),
 // 4/5/6 column kernel with 1/2/4 z cells per thread.
-template <typename T, bool EXACT>
-hipError_t launch_step_impl(const Geo &g, const StepPtrs<T> &p, bool homog, const LaunchCfg &lc)
+// The kernel option that the auto choice (option 0) resolves to.
+template <typename T>
+inline int auto_kernel(const Geo &g, bool homog, int ncu)
 {
+    // the column kernels address each array with 32-bit byte offsets
+    if ((uint64_t)g.ncells * 2u * sizeof(T) >= (1ull << 32)) return 1;
+    if (!homog) {
+        // general path: the y neighbours from LDS (measured 15% faster at
+        // 24^3 x 16^3 in binary64; the homogeneous path is faster without)
+        return g.n[2] % 2 == 0 ? 10 : 9;
+    }
+    if constexpr (sizeof(T) == 4) {
+        // binary32, nx <= 24: pairs with the wave-local neighbour exchange
+        // (24^3 x 16^3: 0.204 against the quad kernel's 0.222 and the pair
+        // kernel's 0.231 ms)
+        if (g.n[0] <= 24 && colx2wl_fits(g, true, 2) && colx2_fills(g, 2, ncu)) return 22;
+        // binary32, wider rows: quads, two z cells per thread (32^3 x 16^3:
+        // 0.430 against the pair kernel's 0.497 and the wave-local pair
+        // kernel's 0.468 ms)
+        if (colq_fits(g, true, 2) && colq_fills(g, 2, ncu)) return 19;
+    }
+    // paired accesses, two z cells per thread where nz is even (binary64
+    // 24^3 x 16^3: 10% faster than the column kernel)
+    if (colx2_fits(g, true, 1)) return g.n[2] % 2 == 0 ? 16 : 15;
+    return g.n[2] % 2 == 0 ? 5 : 4;
+}
+
+template <typename T, bool EXACT>
+hipError_t launch_step_impl(const Geo &g, const StepPtrs<T> &p, bool homog, const LaunchCfg &lc0)
+{
+    LaunchCfg lc = lc0;
+    if (lc.kernel == 0) lc.kernel = auto_kernel<T>(g, homog, lc.ncu);
     if (lc.kernel == 7 || lc.kernel == 8) {
         const bool col_ok = (uint64_t)g.ncells * 2u * sizeof(T) < (1ull << 32);
         if (col_ok && lc.kernel == 8 && plane_fits<T>(g, 2)) return launch_plane<T, EXACT, 2>(g, p, homog, lc.stream);
@@ -688,34 +717,6 @@ hipError_t launch_step_impl(const Geo &g, const StepPtrs<T> &p, bool homog, cons
         if (zc == 4) return launch_colp<T, EXACT, 4>(g, p, homog, lc.stream);
         if (zc == 2) return launch_colp<T, EXACT, 2>(g, p, homog, lc.stream);
         return launch_colp<T, EXACT, 1>(g, p, homog, lc.stream);
-    }
-    if constexpr (sizeof(T) == 4) {
-        if (lc.kernel == 0 && homog && col_fits && g.n[0] <= 24 && colx2wl_fits(g, homog, 2) &&
-            colx2_fills(g, 2, lc.ncu)) {
-            // binary32 homogeneous path, nx <= 24: pairs with the wave-local
-            // neighbour exchange (24^3 x 16^3: 0.204 against the quad kernel's
-            // 0.222 and the pair kernel's 0.231 ms)
-            return launch_colx2<T, EXACT, 2, true>(g, p, lc.stream);
-        }
-        if (lc.kernel == 0 && homog && col_fits && colq_fits(g, homog, 2) && colq_fills(g, 2, lc.ncu)) {
-            // binary32 homogeneous path, wider rows: quads, two z cells per
-            // thread (32^3 x 16^3: 0.430 against the pair kernel's 0.497 and
-            // the wave-local pair kernel's 0.468 ms)
-            return launch_colq<EXACT, 2>(g, p, lc.stream);
-        }
-    }
-    if (lc.kernel == 0 && homog && col_fits && colx2_fits(g, homog, 1)) {
-        // homogeneous path: paired accesses, two z cells per thread (binary64
-        // 24^3 x 16^3: 10% faster than the column kernel; binary32: 3% faster
-        // at 24^3 x 16^3, as fast as four z cells per thread at 32^3 x 16^3)
-        if (g.n[2] % 2 == 0) return launch_colx2<T, EXACT, 2>(g, p, lc.stream);
-        return launch_colx2<T, EXACT, 1>(g, p, lc.stream);
-    }
-    if (lc.kernel == 0 && !homog && col_fits) {
-        // general path: the y neighbours from LDS (measured 15% faster at
-        // 24^3 x 16^3 in binary64; the homogeneous path is faster without)
-        if (g.n[2] % 2 == 0) return launch_col<T, EXACT, 2, 1>(g, p, homog, lc.stream);
-        return launch_col<T, EXACT, 1, 1>(g, p, homog, lc.stream);
     }
     if (lc.kernel != 1 && col_fits) {
         const int zc = col_zc(g, lc.kernel);

@@ -31,7 +31,7 @@ FIELD = {"E": 0, "dE": 1, "meanfl": 2, "omega_i": 3, "omega_e": 4,
          "lambda_x": 17, "lambda_y": 18, "lambda_z": 19, "lambda": 20}
 LAYOUT_INTERIOR, LAYOUT_FACES, LAYOUT_HALO = 0, 1, 2
 OPT_EXACT, OPT_PATH, OPT_BLOCK, OPT_KERNEL = 1, 2, 3, 4
-Q_HOMOGENEOUS, Q_DEVICE_BYTES, Q_CELLS, Q_STEPS, Q_BYTES_PER_CELL = 1, 2, 3, 4, 5
+Q_HOMOGENEOUS, Q_DEVICE_BYTES, Q_CELLS, Q_STEPS, Q_BYTES_PER_CELL, Q_KERNEL = 1, 2, 3, 4, 5, 6
 
 # every symbol include/rt6d.h declares
 EXPORTS = ("rt6d_version", "rt6d_last_error", "rt6d_device_count", "rt6d_create", "rt6d_destroy",
@@ -323,6 +323,11 @@ class Model:
     @property
     def bytes_per_cell(self) -> float:
         return self.query(Q_BYTES_PER_CELL) / 1000.0
+
+    @property
+    def kernel(self) -> int:
+        """The step kernel option rt6d_step runs (the auto choice resolved)."""
+        return self.query(Q_KERNEL)
 
     def synchronize(self):
         _check(self.lib, self.lib.rt6d_synchronize(self._h), "rt6d_synchronize")

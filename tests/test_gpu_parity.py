@@ -240,6 +240,28 @@ def test_fp32_quad_kernel_against_reference(entry, kernel, gpu_available):
     assert rel_l2(outs[0], g["EN"]) <= TOL32
 
 
+@pytest.mark.parametrize("n, precision, path, expected", [
+    ((24, 24, 24, 16, 16, 16), rt6d.FP64, 0, 16),     # pairs, two z cells
+    ((24, 24, 24, 16, 16, 16), rt6d.FP32, 0, 22),     # wave-local pairs
+    ((32, 32, 32, 16, 16, 16), rt6d.FP32, 0, 19),     # quads
+    ((12, 12, 12, 8, 8, 8), rt6d.FP32, 0, 16),        # too small to fill: pairs
+    ((24, 24, 23, 16, 16, 16), rt6d.FP32, 0, 15),     # odd nz: pairs, one z cell
+    ((23, 24, 24, 8, 8, 8), rt6d.FP64, 0, 5),         # odd nx: column kernel
+    ((24, 24, 24, 8, 8, 8), rt6d.FP64, 1, 10),        # general path: LDS-staged y
+], ids=["fp64-24", "fp32-24", "fp32-32", "fp32-12", "fp32-oddz", "fp64-oddx", "general"])
+def test_auto_kernel_choice(n, precision, path, expected, gpu_available):
+    """RT6D_Q_KERNEL reports what the auto choice resolves to for the grid
+    (the choice is invisible in the results, so it is checked here)."""
+    case = cases.member_spec(3, 0, n=n)
+    with rt6d.Model.from_case(case, precision=precision, path=path) as m:
+        m.calc_parameter()
+        assert m.homogeneous == (path == 0)
+        assert m.kernel == expected
+    with rt6d.Model.from_case(case, precision=precision, kernel=4) as m:
+        m.calc_parameter()
+        assert m.kernel == 4
+
+
 # ----------------------------------------------------------- production size
 
 @pytest.fixture(scope="module")
