@@ -82,16 +82,18 @@ extern "C" {
                                 y neighbour from LDS (nx*ny <= 1024);
                                 15/16/17 column kernel with two x cells per thread (paired
                                 accesses), 1/2/4 z cells per thread: homogeneous path with
-                                nx even, else the default;
+                                nx even;
                                 18/19/20 column kernel with four x cells per thread
                                 (16-byte accesses), 1/2/4 z cells per thread: binary32
-                                homogeneous path with nx divisible by 4, else the default;
+                                homogeneous path with nx divisible by 4;
                                 21/22/23 the pair kernel with the x and y neighbour pairs
                                 exchanged through wave-local LDS slots: homogeneous path,
-                                nx even and <= 32, else the default;
+                                nx even and <= 32;
                                 24/25/26 the quad kernel with the same exchange: binary32
-                                homogeneous path, nx divisible by 4 and <= 64, else the
-                                default */
+                                homogeneous path, nx divisible by 4 and <= 64.
+                                An option whose kernel does not fit the grid runs the
+                                column kernel (two z cells per thread when nz is even), or
+                                the per-cell kernel on grids whose arrays exceed 4 GiB */
 
 /* queries for rt6d_query */
 #define RT6D_Q_HOMOGENEOUS 1 /* 1 if the horizontally homogeneous path is active */
