@@ -93,7 +93,7 @@ extern "C" {
                                 homogeneous path, nx divisible by 4 and <= 64.
                                 An option whose kernel does not fit the grid runs the
                                 column kernel (two z cells per thread when nz is even), or
-                                the per-cell kernel on grids whose arrays exceed 4 GiB */
+                                the per-cell kernel where one field reaches 2 GiB */
 
 /* queries for rt6d_query */
 #define RT6D_Q_HOMOGENEOUS 1 /* 1 if the horizontally homogeneous path is active */
