@@ -77,7 +77,7 @@ struct ColX2Cfg {
 // Wave-local neighbour exchange (WL): each wavefront of the block keeps, per
 // z cell, 128 pair slots in LDS: H = nx pairs (two rows) before its 64 pairs,
 // its own pairs, and H after (nx <= 32).  Every step a wave writes its own
-// pairs and loads the halo pairs once (one 16-byte access by lanes 0..2H-1),
+// pairs and loads the halo pairs once (one pair access by lanes 0..2H-1),
 // then reads the x and y neighbour pairs from the slots.  Only the wave's own
 // lanes touch its slots, and the LDS executes a wave's accesses in order, so
 // no barrier is needed; wave_barrier keeps the compiler from moving the
