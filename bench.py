@@ -130,17 +130,18 @@ def _traffic(workload, tag):
         return None
 
 
-def _step_kernel(precision, homog, nz):
-    """Name of the kernel one default step launches at this workload (kernel
-    option 0: the column kernel, two z cells per thread when nz is even,
-    compiled for all advected axes active, y neighbours from LDS on the
-    general path)."""
-    zc = 2 if nz % 2 == 0 else 1
-    if homog and precision == "double":
-        return f"k_step_colx2<double, false, {zc}, true>"
-    h = "true" if homog else "false"
-    ym = 0 if homog else 1
-    return f"k_step_col<{precision}, false, {h}, {zc}, true, {ym}>"
+def _step_kernel(precision, option):
+    """Name of the kernel one step launches for a resolved kernel option
+    (rt6d_query RT6D_Q_KERNEL), compiled for all advected axes active."""
+    T = precision
+    names = {
+        15: f"k_step_colx2<{T}, false, 1, true>", 16: f"k_step_colx2<{T}, false, 2, true>",
+        21: f"k_step_colx2<{T}, false, 1, true, true>", 22: f"k_step_colx2<{T}, false, 2, true, true>",
+        18: "k_step_colq<false, 1, true>", 19: "k_step_colq<false, 2, true>",
+        9: f"k_step_col<{T}, false, false, 1, true, 1>", 10: f"k_step_col<{T}, false, false, 2, true, 1>",
+        4: f"k_step_col<{T}, false, true, 1, true, 0>", 5: f"k_step_col<{T}, false, true, 2, true, 0>",
+    }
+    return names.get(option, f"kernel option {option}")
 
 
 def main():
@@ -181,6 +182,7 @@ def main():
             "launch_ms": ev_ms / args.steps,
             "bytes_per_cell": m.bytes_per_cell,
             "homogeneous": m.homogeneous,
+            "kernel_option": m.kernel,
             "dt": dt,
         }
         res["achieved_GBs"] = res["bytes_per_cell"] * c.cells / (res["launch_ms"] * 1e-3) / 1e9
@@ -223,7 +225,7 @@ def main():
             extras["serving"] = serve.serve_benchmark(40.0, args.serve_seconds, 200.0, baseline_seconds=10.0,
                                                       log=_log)
 
-    kname = _step_kernel("double", main_res["homogeneous"], case.n[2])
+    kname = _step_kernel("double", main_res["kernel_option"])
     ttag = "fp64_auto" if main_res["homogeneous"] else "fp64_general"
     achieved = main_res["achieved_GBs"]
     roofline = {
