@@ -76,6 +76,35 @@ def test_ensemble_equals_individual_runs(precision, gpu_available):
         m.close()
 
 
+@pytest.mark.parametrize("precision", [rt6d.FP64, rt6d.FP32], ids=["fp64", "fp32"])
+def test_ensemble_member_kernel_variants(precision, gpu_available):
+    """Members that take each ensemble kernel variant of the homogeneous path
+    -- in binary32 the wave-local pair kernel (nx <= 24), the quad kernel
+    (nx > 24, nx divisible by 4) and plain pairs (nx > 24, nx not divisible
+    by 4); in binary64 pairs with one and two z cells -- step together and
+    each equals its own run bit for bit, through the graph and the single
+    launches, in both block modes."""
+    cs = [cases.packet_case(n=(8, 8, 4, 4, 4, 4), name="nx8"),
+          cases.packet_case(n=(28, 4, 6, 4, 3, 4), name="nx28"),
+          cases.packet_case(n=(26, 3, 4, 3, 3, 4), name="nx26"),
+          cases.packet_case(n=(32, 5, 5, 3, 4, 4), name="nx32_odd_nz")]
+    ms = [_model(c, precision) for c in cs]
+    assert all(m.homogeneous for m in ms)
+    with rt6d.Ensemble(precision) as ens:
+        for m in ms:
+            ens.add(m)
+        ens.set_schedule(0, 0)
+        ens.step(3)
+        ens.set_schedule(4, 1)
+        ens.step(4)
+        for c, m in zip(cs, ms):
+            E = m.download("E")
+            ref = _alone(c, precision, 7)
+            assert np.array_equal(E, ref), f"{c.name}: max diff {np.max(np.abs(E - ref))}"
+    for m in ms:
+        m.close()
+
+
 def test_ensemble_tune_and_state(gpu_available):
     cs = _cases()[:3]
     ms = [_model(c, rt6d.FP64) for c in cs]
