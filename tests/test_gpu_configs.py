@@ -1,0 +1,159 @@
+"""The production workloads at their own sizes, through the C ABI.
+
+* every golden entry: the device CFL dt equals the fixture's dt, which came
+  from the reference's own velocities;
+* 32^3 x 16^3 (134,217,728 cells, the FP32 scaling grid), both precisions:
+  determinism, every step kernel bit for bit the same in exact mode, the
+  FP32 form within 1e-5 of binary64, and the energy budget;
+* an ensemble of generator members at their drawn sizes ({12..24}^3 x
+  {8..16}^3), in both precisions: each member equals its own run bit for bit,
+  and a 12^3 x 8^3 member equals the oracle (binary64 <= 1e-12, FP32 <= 1e-5);
+* a Poisson request stream of generator members (steps U[5, 20], FP32:FP64
+  3:1) served against an HBM budget that makes requests wait: each result
+  equals the request run alone, bit for bit.
+"""
+import numpy as np
+import pytest
+
+from conftest import golden_case, golden_data, golden_index, rel_l2
+
+import oracle as O
+from raytracer_3d_amd import cases, rt6d, serve
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 1e-5
+ENTRIES = golden_index()
+
+
+@pytest.mark.parametrize("entry", ENTRIES, ids=[e["stem"] for e in ENTRIES])
+def test_device_cfl_dt_equals_golden_dt(entry, gpu_available):
+    case = golden_case(entry)
+    g = golden_data(entry)
+    with rt6d.Model.from_case(case) as m:
+        m.calc_parameter()
+        assert m.cfl_dt(0.5) == float(g["dt"])
+
+
+def _budget_ok(d):
+    terms = [d["eint"], d["mflint"]] + [d[k] for k in d if k.startswith("bfl_")]
+    return abs(d["residual"]) <= 1e-11 * sum(abs(t) for t in terms)
+
+
+def test_32cube_both_precisions(gpu_available):
+    c = cases.gm_case(n=(32, 32, 32, 16, 16, 16))
+    print(f"[32cube] {c.n}", flush=True)
+    with rt6d.Model.from_case(c) as m:
+        m.calc_parameter()
+        assert m.homogeneous
+        dt = m.cfl_dt(0.5)
+        m.integrate(10, dt=dt)
+        e64 = m.download("E")
+        d = m.diagnose(dt=dt)
+    assert np.all(np.isfinite(e64)) and _budget_ok(d)
+    with rt6d.Model.from_case(c) as m:
+        m.calc_parameter()
+        m.integrate(10, dt=dt)
+        assert np.array_equal(m.download("E"), e64), "not deterministic"
+    ex = None
+    for kernel in (0, 1, 5, 16, 22):
+        with rt6d.Model.from_case(c, exact=True, kernel=kernel) as m:
+            m.calc_parameter()
+            m.integrate(10, dt=dt)
+            e = m.download("E")
+        if ex is None:
+            ex = e
+            assert rel_l2(ex, e64) <= TOL64
+        else:
+            assert np.array_equal(e, ex), f"kernel {kernel} differs in exact mode"
+    outs = {}
+    for kernel in (0, 5, 19):
+        with rt6d.Model.from_case(c, precision=rt6d.FP32, kernel=kernel) as m:
+            m.calc_parameter()
+            m.integrate(10, dt=dt)
+            outs[kernel] = m.download("E")
+            if kernel == 0:
+                d32 = m.diagnose(dt=dt)
+    assert np.array_equal(outs[0], outs[5]) and np.array_equal(outs[19], outs[5])
+    assert rel_l2(outs[0], e64) <= TOL32
+    assert abs(d32["eint"] - d["eint"]) <= 1e-4 * abs(d["eint"])
+
+
+SMALL = (12, 12, 12, 8, 8, 8)
+
+
+def _members():
+    """Six generator members at their drawn sizes plus the smallest grid."""
+    specs = [cases.member_spec(4242, i) for i in range(6)]
+    specs.append(cases.member_spec(4242, 90, n=SMALL))
+    return specs
+
+
+def _oracle_E(spec, dt, nsteps):
+    c = spec
+    i = c.meta["init"]
+    c.E0 = cases.separable_E(i["factors"], i["peak"], i["noise"], i["seed"], i["member"])
+    c.dt = dt
+    o = O.Oracle(c)
+    o.calc_parameter()
+    o.integrate(nsteps)
+    E = o.get("E")
+    o.close()
+    return E
+
+
+@pytest.mark.parametrize("precision", [rt6d.FP64, rt6d.FP32], ids=["fp64", "fp32"])
+def test_generator_member_ensemble(precision, gpu_available):
+    specs = _members()
+    print("[ensemble] " + " ".join("x".join(map(str, s.n)) for s in specs), flush=True)
+    ms = []
+    for s in specs:
+        m = rt6d.Model.from_case(s, precision=precision)
+        m.calc_parameter()
+        m.dt = m.cfl_dt(0.5)
+        ms.append(m)
+    try:
+        with rt6d.Ensemble(precision) as ens:
+            for m in ms:
+                ens.add(m)
+            ens.step(3)
+            ens.set_schedule(4, 1)
+            ens.step(4)
+            assert ens.info()["steps"] == 7
+            got = [m.download("E") for m in ms]
+        for s, m, E in zip(specs, ms, got):
+            with rt6d.Model.from_case(s, precision=precision) as solo:
+                solo.calc_parameter()
+                assert solo.cfl_dt(0.5) == m.dt
+                solo.integrate(7, dt=m.dt)
+                assert np.array_equal(E, solo.download("E")), f"{s.n} differs from its own run"
+        ref = _oracle_E(specs[-1], ms[-1].dt, 7)
+        err = rel_l2(got[-1], ref)
+        assert err <= (TOL64 if precision == rt6d.FP64 else TOL32), err
+    finally:
+        for m in ms:
+            m.close()
+
+
+def test_request_stream_mix_equals_alone(gpu_available):
+    reqs = serve.request_stream(40.0, 0.3, seed=4242, steps_range=(5, 20))
+    assert len(reqs) >= 8
+    print("[serve] " + " ".join(f"{r.rid}:{'x'.join(map(str, r.n))}/{r.steps}/{r.precision}" for r in reqs),
+          flush=True)
+    assert {r.precision for r in reqs} == {rt6d.FP32, rt6d.FP64}
+    # room for about a third of the requests at once: the rest wait for HBM
+    peaks = sorted(serve.peak_bytes(r.n, r.precision) for r in reqs)
+    budget = max(peaks[-1], sum(peaks) / 3)
+    srv = serve.Server(budget, chunk=4, keep_E=True, tune=False)
+    try:
+        srv.run(reqs)
+        assert srv.peak_used <= budget
+        assert srv.used == pytest.approx(0.0, abs=1.0)
+    finally:
+        srv.close()
+    assert any(r.admitted > r.arrival + 1e-3 for r in reqs), "no request waited for room"
+    for r in reqs:
+        assert r.result is not None and r.result["steps"] == r.steps
+        _, E = serve.run_alone(r)
+        assert np.array_equal(r.result["E"], E), f"request {r.rid} ({r.n}, prec {r.precision}) differs"
