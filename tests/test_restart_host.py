@@ -73,3 +73,57 @@ def test_reader_refuses_what_read_restart_refuses(tmp_path):
             f.write(np.array([len(payload)], "<i4").tobytes() + payload + np.array([len(payload)], "<i4").tobytes())
     with pytest.raises(ValueError, match="PE boundaries"):
         restart.read_restart(q, N)
+
+
+def _gfortran_subrecords(payload, limit):
+    """gfortran's framing of one record, assembled here from its rule: the
+    leading marker of every subrecord but the last is negative, the trailing
+    marker of every subrecord but the first is negative."""
+    parts = [payload[i:i + limit] for i in range(0, len(payload), limit)] or [b""]
+    out = b""
+    for k, p in enumerate(parts):
+        lead = -len(p) if k < len(parts) - 1 else len(p)
+        trail = -len(p) if k > 0 else len(p)
+        out += np.array([lead], "<i4").tobytes() + p + np.array([trail], "<i4").tobytes()
+    return out
+
+
+@pytest.mark.parametrize("limit", [100, 8 * 37, 977])
+def test_long_records_split_into_gfortran_subrecords(tmp_path, limit):
+    """Records longer than the subrecord limit (2,147,483,639 bytes by
+    default, which the field record passes from prod(n+4) > 89.5M) are split
+    as gfortran splits them; a small limit checks the split byte for byte,
+    including a limit that cuts a binary64 value and one that divides the
+    record, and the reader joins the subrecords again."""
+    E, dE, mf = _coded(N, 1e6), _coded(N, 2e6), _coded(N, 3e6)
+    p = tmp_path / "r.dta"
+    restart.write_restart(str(p), N, 5, E, dE, mf, ritt=False, max_subrecord=limit)
+    H = [v + 4 for v in N]
+    expect = b""
+    for payload in (np.array(list(N) + [5], "<i4").tobytes(),
+                     np.array([1, N[0], 1, N[1], 1, N[2]], "<i4").tobytes(),
+                     np.array([1, N[3], 1, N[4], 1, N[5]], "<i4").tobytes()):
+        expect += _gfortran_subrecords(payload, limit)
+    body = b""
+    for a in (E, dE, mf):
+        h = np.zeros(H, order="F")
+        h[2:-2, 2:-2, 2:-2, 2:-2, 2:-2, 2:-2] = a.reshape(N, order="F")
+        body += h.tobytes(order="F")
+    expect += _gfortran_subrecords(body, limit)
+    data = p.read_bytes()
+    assert data == expect
+    assert np.frombuffer(data[:4], "<i4")[0] == 28          # short records stay whole
+    r = restart.read_restart(str(p), N)
+    assert r["itt"] == 5
+    assert np.array_equal(r["E"], E) and np.array_equal(r["dE"], dE) and np.array_equal(r["meanfl"], mf)
+    # a continuation flag that contradicts the framing is refused
+    bad = bytearray(data)
+    at = len(data) - len(_gfortran_subrecords(body, limit))
+    bad[at:at + 4] = np.array([abs(int(np.frombuffer(data[at:at + 4], "<i4")[0]))], "<i4").tobytes()
+    (tmp_path / "bad.dta").write_bytes(bytes(bad))
+    with pytest.raises(ValueError, match="framing|records"):
+        restart.read_restart(str(tmp_path / "bad.dta"))
+
+
+def test_default_limit_is_gfortrans():
+    assert restart.MAX_SUBRECORD == 2 ** 31 - 9
