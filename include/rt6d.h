@@ -94,6 +94,11 @@ extern "C" {
                                 An option whose kernel does not fit the grid runs the
                                 column kernel (two z cells per thread when nz is even), or
                                 the per-cell kernel where one field reaches 2 GiB */
+#define RT6D_OPT_SYNC 5      /* 1 (default): rt6d_step returns once its steps are done;
+                                0: rt6d_step returns once they are enqueued on the context's
+                                stream, and the next call that reads the device state
+                                (download, diagnose, rt6d_synchronize, ...) waits for them;
+                                a fault of those steps is reported by that call */
 
 /* queries for rt6d_query */
 #define RT6D_Q_HOMOGENEOUS 1 /* 1 if the horizontally homogeneous path is active */

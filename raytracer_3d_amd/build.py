@@ -3,7 +3,8 @@
 * ``librt6d.so`` -- the HIP library (raytracer_3d_amd/csrc/*) for gfx950,
   exporting the C ABI of include/rt6d.h.
 * ``fortran/`` -- the Fortran drop-in driver linked against librt6d.so
-  (see raytracer_3d_amd/fortran/Makefile).
+  (see raytracer_3d_amd/fortran/Makefile), with the package's case-file
+  plug-in or with a config plug-in of the reference's form.
 
 Run ``python -m raytracer_3d_amd.build`` or call :func:`build_all`.
 """
@@ -81,6 +82,21 @@ def build_fortran(verbose: bool = False) -> str | None:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("fortran build failed")
     return os.path.join(fdir, "rt6d_driver")
+
+
+def build_plugin(src: str, outdir: str, mpi: bool = False) -> str | None:
+    """Build the Fortran driver around a config plug-in (the reference's
+    for_config/*.f90 form) as outdir/model_<name>.x; None without amdflang."""
+    fdir = os.path.join(PKG, "fortran")
+    if shutil.which("amdflang") is None and not os.path.exists("/opt/rocm/bin/amdflang"):
+        return None
+    cmd = ["make", "-s", "-C", fdir, "plugin", f"PLUGIN={os.path.abspath(src)}",
+           f"PLUGIN_DIR={os.path.abspath(outdir)}", f"MPI={1 if mpi else 0}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("plug-in build failed: " + src)
+    return os.path.join(outdir, "model_" + os.path.splitext(os.path.basename(src))[0] + ".x")
 
 
 def build_all(verbose: bool = False) -> None:

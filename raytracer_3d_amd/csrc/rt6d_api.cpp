@@ -86,7 +86,7 @@ struct rt6d_ctx {
     size_t vel_count[kAxes] = {0};
     bool params_ready = false;
     bool homog = false;
-    int opt_exact = 0, opt_path = 0, opt_block = 256, opt_kernel = 0;
+    int opt_exact = 0, opt_path = 0, opt_block = 256, opt_kernel = 0, opt_sync = 1;
     void *d_hx = nullptr, *d_hy = nullptr, *d_hz = nullptr, *d_hm = nullptr, *d_hl = nullptr;
     uint32_t n4 = 0, inner4m = 0;
     double vmax[kAxes] = {0};
@@ -825,7 +825,7 @@ int rt6d_step(rt6d_ctx *c, double dt, int nsteps)
     int r;
     if ((r = check_ctx(c))) return r;
     if ((r = do_steps(c, dt, nsteps))) return r;
-    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->opt_sync) HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }
 
@@ -1087,6 +1087,10 @@ int rt6d_set_option(rt6d_ctx *c, int option, int value)
         if ((value == 6 || value == 11) && c->n[2] % 4 != 0)
             return fail(RT6D_EINVAL, "four z cells per thread need nz divisible by 4 (have %d)", c->n[2]);
         c->opt_kernel = value;
+        return 0;
+    case RT6D_OPT_SYNC:
+        if (value < 0 || value > 1) return fail(RT6D_EINVAL, "sync option %d", value);
+        c->opt_sync = value;
         return 0;
     default:
         return fail(RT6D_EINVAL, "unknown option %d", option);

@@ -29,7 +29,7 @@ def run(case, nsteps: int | None = None, runlen: float | None = None, snapint: f
         precision=rt6d.FP64, directory: str = ".", use_restart: bool = False, write_6d: bool = True,
         forcing: tuple | None = None, cfl: float = 0.5, log=None) -> dict:
     """Integrate `case` as main.f90 does. The step count is nsteps, else
-    int(runlen/dt + 0.5), else case.nsteps. dt is case.dt, else the CFL
+    int(runlen/dt) (main.f90:51), else case.nsteps. dt is case.dt, else the CFL
     choice. forcing: (kind, field) for rt6d_set_forcing. Returns itt, the
     steps taken, the step-loop seconds and rate, and the snapshot count."""
     os.makedirs(directory, exist_ok=True)
@@ -49,7 +49,7 @@ def run(case, nsteps: int | None = None, runlen: float | None = None, snapint: f
                 itt = _restart.load(m, rfile)
             else:
                 say(f"found no restart file {rfile}")
-        steps = nsteps if nsteps is not None else (int(runlen / dt + 0.5) if runlen is not None else case.nsteps)
+        steps = nsteps if nsteps is not None else (int(runlen / dt) if runlen is not None else case.nsteps)
         enditt = itt + steps
         snapsteps = max(1, int(snapint / dt)) if snapint else 0
         snaps = snapshot.Snapshots(m, case, directory, write_6d=write_6d) if snapsteps else None

@@ -2,7 +2,9 @@
 ! e.g. for_config/gm1.f90:7-121) that takes its integration from a case file
 ! written by raytracer_3d_amd/cases.py: int32[16] header, float64[8]
 ! parameters, six interior T-spacing vectors, Nsqr/U/V halo'd, interior E.
-! The file name is the driver's first command-line argument.
+! The file name is the driver's first command-line argument; an argument
+! snap=N asks for diagnose every N steps.  The forcing of a case, if any,
+! is set on the device (rt6d_set_forcing), so E stays resident there.
 module config_case_state
   implicit none
   integer :: case_unit = -1
@@ -13,11 +15,16 @@ subroutine set_parameter
   use main_module
   use config_case_state
   implicit none
-  integer :: hdr(16)
+  integer :: hdr(16), i, snapsteps
   real*8 :: rh(8)
-  character(len=1024) :: fname
+  character(len=1024) :: fname, arg
   call get_command_argument(1, fname)
-  if (len_trim(fname) == 0) call halt_stop(' usage: rt6d_driver CASE [OUT] [SNAPSTEPS] [fp32]')
+  if (len_trim(fname) == 0) call halt_stop(' usage: rt6d_driver CASE [snap=N] [out=FILE] [fp32] [restart]')
+  snapsteps = 0
+  do i = 2, command_argument_count()
+    call get_command_argument(i, arg)
+    if (arg(1:5) == 'snap=') read(arg(6:), *) snapsteps
+  enddo
   open(newunit=case_unit, file=trim(fname), access='stream', form='unformatted', status='old', action='read')
   read(case_unit) hdr
   if (hdr(1) /= 1144149074) call halt_stop(' not a case file')
@@ -28,7 +35,12 @@ subroutine set_parameter
   case_steps = hdr(11)
   dt = rh(1); f0 = rh(2); sign_of_omega_i = rh(3)
   k_fixed = rh(4); l_fixed = rh(5); m_fixed = rh(6)
-  runlen = case_steps*dt
+  ! main.f90:51 takes int(runlen/dt) steps and main.f90:74 diagnoses every
+  ! int(snapint/dt) steps: half a step more keeps both whole step counts
+  runlen = (case_steps + 0.5d0)*dt
+  snapint = 0
+  if (snapsteps > 0) snapint = (snapsteps + 0.5d0)*dt
+  host_forcing = .false.
 end subroutine set_parameter
 
 subroutine set_grid

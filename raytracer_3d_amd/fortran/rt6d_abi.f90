@@ -10,7 +10,8 @@ module rt6d_abi
   integer(c_int), parameter :: RT6D_FIELD_E = 0, RT6D_FIELD_DE = 1, RT6D_FIELD_MEANFL = 2
   integer(c_int), parameter :: RT6D_FIELD_OMEGA_I = 3, RT6D_FIELD_OMEGA_E = 4
   integer(c_int), parameter :: RT6D_FIELD_XDOT = 5, RT6D_FIELD_FLUX_X = 11, RT6D_FIELD_LAMBDA_X = 17
-  integer(c_int), parameter :: RT6D_OPT_EXACT = 1, RT6D_OPT_PATH = 2, RT6D_OPT_BLOCK = 3, RT6D_OPT_KERNEL = 4
+  integer(c_int), parameter :: RT6D_OPT_EXACT = 1, RT6D_OPT_PATH = 2, RT6D_OPT_BLOCK = 3, RT6D_OPT_KERNEL = 4, &
+                                RT6D_OPT_SYNC = 5
   integer(c_int), parameter :: RT6D_Q_HOMOGENEOUS = 1, RT6D_Q_DEVICE_BYTES = 2
 
   ! scalars of the diagnose reductions (rt6d_diag)

@@ -62,6 +62,9 @@ subroutine allocate_main_module
   up = merge(1, 0, enable_upper_reflection)
   down = merge(1, 0, enable_lower_reflection)
   call rt6d_check(rt6d_create(n, up, down, int(step_precision, c_int), rt6d_handle), 'allocate_main_module')
+  ! integrate enqueues its step and returns; the next call that reads the
+  ! device state (a download, diagnose, rt6d_synchronize) waits for it
+  call rt6d_check(rt6d_set_option(rt6d_handle, RT6D_OPT_SYNC, 0_c_int), 'allocate_main_module')
 end subroutine allocate_main_module
 
 ! u-centred grid from T spacings (calc_grid.f90:300-322 semantics): the U
@@ -140,11 +143,14 @@ contains
   end subroutine axis_grid
 end subroutine calc_grid
 
-! Host E -> device (after set_initial_conditions, a forcing or a restart).
+! Host E -> device (after set_initial_conditions, a host set_forcing or a
+! restart).  Refused while the host E is older than the device E (steps
+! taken since the last e_to_host): uploading it would throw those steps away.
 subroutine e_to_device
   use main_module
   use rt6d_abi
   implicit none
+  if (.not. host_E_current) call halt_stop(' e_to_device: the host E is stale (call e_to_host first)')
   call rt6d_check(rt6d_upload_E(rt6d_handle, E, RT6D_LAYOUT_HALO), 'e_to_device')
   host_E_current = .true.
 end subroutine e_to_device
@@ -175,10 +181,37 @@ subroutine calc_parameter
   call rt6d_check(rt6d_set_grid(h, 5_c_int, mt, dmt), 'calc_parameter/m grid')
   call rt6d_check(rt6d_set_background(h, Nsqr, U, V, f0, sign_of_omega_i), 'calc_parameter/background')
   call rt6d_check(rt6d_calc_parameter(h), 'calc_parameter')
+  if (host_forcing) then
+    call velocity_to_host(0, xdot)
+    call velocity_to_host(1, ydot)
+    call velocity_to_host(2, zdot)
+    call velocity_to_host(3, kdot)
+    call velocity_to_host(4, ldot)
+    call velocity_to_host(5, mdot)
+  endif
   call e_to_device
+contains
+  ! face velocities of axis d on the host, halo'd as the reference allocates
+  ! them; faces 0..n of the axis are filled (face f lies between cells f and
+  ! f+1), the ghost faces stay zero
+  subroutine velocity_to_host(d, a)
+    integer, intent(in) :: d
+    real*8, allocatable, intent(inout) :: a(:,:,:,:,:,:)
+    real*8, allocatable :: tmp(:,:,:,:,:,:)
+    integer :: lo(6)
+    lo = 1
+    lo(d + 1) = 0
+    if (.not. allocated(a)) allocate(a, mold=E)
+    a = 0
+    allocate(tmp(lo(1):nx, lo(2):ny, lo(3):nz, lo(4):nk, lo(5):nl, lo(6):nm))
+    call rt6d_check(rt6d_download(h, RT6D_FIELD_XDOT + d, RT6D_LAYOUT_FACES, tmp), 'calc_parameter/velocities')
+    a(lo(1):nx, lo(2):ny, lo(3):nz, lo(4):nk, lo(5):nl, lo(6):nm) = tmp
+    deallocate(tmp)
+  end subroutine velocity_to_host
 end subroutine calc_parameter
 
-! integrate.f90:4 -- one forward-Euler step of E on the device.
+! integrate.f90:4 -- one forward-Euler step of E on the device, enqueued on
+! the context's stream (allocate_main_module turns the wait off).
 subroutine integrate
   use main_module
   use rt6d_abi
@@ -236,27 +269,140 @@ subroutine restart_file_name(fname)
   enddo
 end subroutine restart_file_name
 
-! write_restart (for_src/restart.f90:1-30): the same four unformatted
-! records -- dimensions and itt, the PE bounds (two records), E, dE, meanfl
-! over their halo'd extents -- and the 'ritt' file.  E comes from the device;
-! dE and meanfl are the last step's by-products (zero before the first step).
+! Records of gfortran's unformatted sequential layout (the reference's
+! compiler), written and read through stream access so that records past
+! 2 GiB keep that layout: subrecords of at most restart_max_subrecord bytes
+! between 4-byte markers; the leading marker is negative when the record
+! continues in the next subrecord, the trailing marker when the subrecord
+! continues a previous one.  A record's payload is the bytes of np pieces.
+subroutine put_record(io, np, p, nb)
+  use iso_c_binding
+  use main_module, only: restart_max_subrecord
+  implicit none
+  integer, intent(in) :: io, np
+  type(c_ptr), intent(in) :: p(np)
+  integer(c_int64_t), intent(in) :: nb(np)
+  integer(c_int8_t), pointer :: b(:)
+  integer(c_int64_t) :: left, sub, need, take, off
+  integer(c_int32_t) :: mk
+  integer :: ip
+  logical :: first, more
+  left = sum(nb)
+  first = .true.
+  ip = 1
+  off = 0
+  do
+    sub = min(left, restart_max_subrecord)
+    more = left > sub
+    mk = int(sub, c_int32_t)
+    if (more) mk = -mk
+    write(io) mk
+    need = sub
+    do while (need > 0)
+      do while (nb(ip) - off == 0)
+        ip = ip + 1
+        off = 0
+      enddo
+      take = min(need, nb(ip) - off)
+      call c_f_pointer(p(ip), b, [nb(ip)])
+      write(io) b(off + 1:off + take)
+      off = off + take
+      need = need - take
+    enddo
+    mk = int(sub, c_int32_t)
+    if (.not. first) mk = -mk
+    write(io) mk
+    left = left - sub
+    first = .false.
+    if (.not. more) exit
+  enddo
+end subroutine put_record
+
+! One record of that layout into np pieces; ok is false when the framing is
+! broken or the record's length is not the pieces' total.
+subroutine get_record(io, np, p, nb, ok)
+  use iso_c_binding
+  implicit none
+  integer, intent(in) :: io, np
+  type(c_ptr), intent(in) :: p(np)
+  integer(c_int64_t), intent(in) :: nb(np)
+  logical, intent(out) :: ok
+  integer(c_int8_t), pointer :: b(:)
+  integer(c_int64_t) :: total, got, sub, need, take, off
+  integer(c_int32_t) :: lead, trail
+  integer :: ip, ios
+  logical :: first
+  ok = .false.
+  total = sum(nb)
+  got = 0
+  first = .true.
+  ip = 1
+  off = 0
+  do
+    read(io, iostat=ios) lead
+    if (ios /= 0) return
+    sub = abs(int(lead, c_int64_t))
+    if (got + sub > total) return
+    need = sub
+    do while (need > 0)
+      do while (nb(ip) - off == 0)
+        ip = ip + 1
+        off = 0
+      enddo
+      take = min(need, nb(ip) - off)
+      call c_f_pointer(p(ip), b, [nb(ip)])
+      read(io, iostat=ios) b(off + 1:off + take)
+      if (ios /= 0) return
+      off = off + take
+      need = need - take
+    enddo
+    read(io, iostat=ios) trail
+    if (ios /= 0) return
+    if (abs(int(trail, c_int64_t)) /= sub .or. ((trail < 0) .eqv. first)) return
+    got = got + sub
+    first = .false.
+    if (lead >= 0) exit
+  enddo
+  ok = got == total
+end subroutine get_record
+
+! write_restart (for_src/restart.f90:1-30): the same four records --
+! dimensions and itt, the PE bounds (two records), E, dE, meanfl over their
+! halo'd extents -- and the 'ritt' file.  E comes from the device; dE and
+! meanfl are the last step's by-products (zero before the first step).
 subroutine write_restart
+  use iso_c_binding
   use main_module
   use rt6d_abi
   implicit none
   character(len=80) :: fname
-  integer :: io
+  integer :: io, ios
+  integer(c_int32_t), target :: h1(7), h2(6), h3(6)
+  integer(c_int64_t) :: nbf
+  interface
+    subroutine put_record(io, np, p, nb)
+      import :: c_ptr, c_int64_t
+      integer, intent(in) :: io, np
+      type(c_ptr), intent(in) :: p(np)
+      integer(c_int64_t), intent(in) :: nb(np)
+    end subroutine
+  end interface
   call e_to_host
   call rt6d_check(rt6d_download(rt6d_handle, RT6D_FIELD_DE, RT6D_LAYOUT_HALO, dE), 'write_restart/dE')
   call rt6d_check(rt6d_download(rt6d_handle, RT6D_FIELD_MEANFL, RT6D_LAYOUT_HALO, meanfl), 'write_restart/meanfl')
   call restart_file_name(fname)
   print '(a,a)', ' writing to restart file ', trim(fname)
   print '(a,i12)', ' at itt = ', itt
-  open(newunit=io, file=trim(fname), form='unformatted', status='unknown', err=10)
-  write(io, err=10) nx, ny, nz, nk, nl, nm, itt
-  write(io, err=10) xs_pe, xe_pe, ys_pe, ye_pe, zs_pe, ze_pe
-  write(io, err=10) ks_pe, ke_pe, ls_pe, le_pe, ms_pe, me_pe
-  write(io, err=10) E, dE, meanfl
+  open(newunit=io, file=trim(fname), access='stream', form='unformatted', status='replace', iostat=ios)
+  if (ios /= 0) goto 10
+  h1 = [nx, ny, nz, nk, nl, nm, itt]
+  h2 = [xs_pe, xe_pe, ys_pe, ye_pe, zs_pe, ze_pe]
+  h3 = [ks_pe, ke_pe, ls_pe, le_pe, ms_pe, me_pe]
+  nbf = 8_c_int64_t*size(E, kind=c_int64_t)
+  call put_record(io, 1, [c_loc(h1)], [28_c_int64_t])
+  call put_record(io, 1, [c_loc(h2)], [24_c_int64_t])
+  call put_record(io, 1, [c_loc(h3)], [24_c_int64_t])
+  call put_record(io, 3, [c_loc(E), c_loc(dE), c_loc(meanfl)], [nbf, nbf, nbf])
   close(io)
   open(newunit=io, file='ritt', form='formatted', status='unknown')
   write(io, *) itt
@@ -271,12 +417,24 @@ end subroutine write_restart
 ! dE and meanfl stay on the host (the device recomputes its by-products at
 ! the next step).
 subroutine read_restart
+  use iso_c_binding
   use main_module
   implicit none
   character(len=80) :: fname
-  logical :: file_exists
-  integer :: io, nx_, ny_, nz_, nk_, nl_, nm_
-  integer :: b(12), bb(12)
+  logical :: file_exists, ok
+  integer :: io, ios
+  integer(c_int32_t), target :: h1(7), b(12)
+  integer :: bb(12)
+  integer(c_int64_t) :: nbf
+  interface
+    subroutine get_record(io, np, p, nb, ok)
+      import :: c_ptr, c_int64_t
+      integer, intent(in) :: io, np
+      type(c_ptr), intent(in) :: p(np)
+      integer(c_int64_t), intent(in) :: nb(np)
+      logical, intent(out) :: ok
+    end subroutine
+  end interface
   call restart_file_name(fname)
   inquire(file=trim(fname), exist=file_exists)
   if (.not. file_exists) then
@@ -284,24 +442,32 @@ subroutine read_restart
     return
   endif
   print '(a,a)', ' reading restart file ', trim(fname)
-  open(newunit=io, file=trim(fname), form='unformatted', status='old', err=10)
-  read(io, err=10) nx_, ny_, nz_, nk_, nl_, nm_, itt
-  if (nx /= nx_ .or. ny /= ny_ .or. nz /= nz_ .or. nk /= nk_ .or. nl /= nl_ .or. nm /= nm_) then
-    print *, ' read from restart dimensions: ', nx_, ny_, nz_, nk_, nl_, nm_
+  open(newunit=io, file=trim(fname), access='stream', form='unformatted', status='old', action='read', iostat=ios)
+  if (ios /= 0) goto 10
+  call get_record(io, 1, [c_loc(h1)], [28_c_int64_t], ok)
+  if (.not. ok) goto 10
+  if (nx /= h1(1) .or. ny /= h1(2) .or. nz /= h1(3) .or. nk /= h1(4) .or. nl /= h1(5) .or. nm /= h1(6)) then
+    print *, ' read from restart dimensions: ', h1(1:6)
     print *, ' does not match dimensions   : ', nx, ny, nz, nk, nl, nm
     call halt_stop(' read_restart')
   endif
-  read(io, err=10) b(1:6)
-  read(io, err=10) b(7:12)
+  itt = h1(7)
+  call get_record(io, 1, [c_loc(b(1))], [24_c_int64_t], ok)
+  if (.not. ok) goto 10
+  call get_record(io, 1, [c_loc(b(7))], [24_c_int64_t], ok)
+  if (.not. ok) goto 10
   bb = [xs_pe, xe_pe, ys_pe, ye_pe, zs_pe, ze_pe, ks_pe, ke_pe, ls_pe, le_pe, ms_pe, me_pe]
   if (any(b /= bb)) then
     print *, ' read from restart PE boundaries: ', b
     print *, ' which does not match           : ', bb
     call halt_stop(' read_restart')
   endif
-  read(io, err=10) E, dE, meanfl
+  nbf = 8_c_int64_t*size(E, kind=c_int64_t)
+  call get_record(io, 3, [c_loc(E), c_loc(dE), c_loc(meanfl)], [nbf, nbf, nbf], ok)
+  if (.not. ok) goto 10
   close(io)
   print *, ' itt = ', itt
+  host_E_current = .true.   ! the file's E is the state from here on
   call e_to_device
   return
 10 continue

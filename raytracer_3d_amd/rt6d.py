@@ -30,7 +30,7 @@ FIELD = {"E": 0, "dE": 1, "meanfl": 2, "omega_i": 3, "omega_e": 4,
          "flux_x": 11, "flux_y": 12, "flux_z": 13, "flux_k": 14, "flux_l": 15, "flux_m": 16,
          "lambda_x": 17, "lambda_y": 18, "lambda_z": 19, "lambda": 20}
 LAYOUT_INTERIOR, LAYOUT_FACES, LAYOUT_HALO = 0, 1, 2
-OPT_EXACT, OPT_PATH, OPT_BLOCK, OPT_KERNEL = 1, 2, 3, 4
+OPT_EXACT, OPT_PATH, OPT_BLOCK, OPT_KERNEL, OPT_SYNC = 1, 2, 3, 4, 5
 Q_HOMOGENEOUS, Q_DEVICE_BYTES, Q_CELLS, Q_STEPS, Q_BYTES_PER_CELL, Q_KERNEL = 1, 2, 3, 4, 5, 6
 
 # every symbol include/rt6d.h declares

@@ -302,17 +302,21 @@ class Server:
 
     def _preempt(self, victim, active, lock):
         prec, q, slot = victim
-        _, m, left = active[prec].pop(slot)
-        self.ens[prec].remove(slot)
+        _, m, left = active[prec][slot]
         done = q.steps - left
+        # checkpoint first: if it fails, the victim is still resident and
+        # in its ensemble, and the server's accounting is unchanged
         if self.checkpoint_dir:
             import os
             from . import restart
             path = os.path.join(self.checkpoint_dir, f"restart_request{q.rid:06d}.dta")
             restart.save(m, path, done, ritt=False)
-            q.resume = {"done": done, "left": left, "path": path}
+            resume = {"done": done, "left": left, "path": path}
         else:
-            q.resume = {"done": done, "left": left, "E": m.download("E")}
+            resume = {"done": done, "left": left, "E": m.download("E")}
+        active[prec].pop(slot)
+        self.ens[prec].remove(slot)
+        q.resume = resume
         m.close()
         with lock:
             self.used -= resident_bytes(q.n, q.precision)

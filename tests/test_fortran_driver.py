@@ -38,7 +38,7 @@ def test_driver_matches_reference(entry, tmp_path, gpu_available):
     cf = str(tmp_path / "c.case")
     of = str(tmp_path / "E.out")
     cases.write_case_file(case, cf, nsteps=int(g["nsteps"]))
-    r = subprocess.run([DRIVER, cf, of, "2"], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    r = subprocess.run([DRIVER, cf, "out=" + of, "snap=2"], capture_output=True, text=True, timeout=120, cwd=tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
     assert (tmp_path / "snap3D.cdf").exists() and (tmp_path / "snap6D.cdf").exists()
     E = np.fromfile(of, dtype="<f8")
@@ -59,7 +59,7 @@ def test_driver_fp32_form(tmp_path, gpu_available):
     cf = str(tmp_path / "c.case")
     of = str(tmp_path / "E.out")
     cases.write_case_file(case, cf, nsteps=int(g["nsteps"]))
-    r = subprocess.run([DRIVER, cf, of, "0", "fp32"], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([DRIVER, cf, "out=" + of, "fp32"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert rel_l2(np.fromfile(of, dtype="<f8"), g["EN"]) <= 1e-5
 
@@ -89,7 +89,7 @@ def test_driver_restart_resumes_bit_for_bit(tmp_path, gpu_available):
     cf3, cf2 = str(tmp_path / "c3.case"), str(tmp_path / "c2.case")
     cases.write_case_file(case, cf3, nsteps=3)
     cases.write_case_file(case, cf2, nsteps=2)
-    r = subprocess.run([DRIVER, cf3, "E3.out", "0", "fp64", "restart"], capture_output=True, text=True,
+    r = subprocess.run([DRIVER, cf3, "out=E3.out", "restart"], capture_output=True, text=True,
                        timeout=120, cwd=tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
     rec = _fortran_records(tmp_path / "restart_pe=000000000.dta")
@@ -111,7 +111,7 @@ def test_driver_restart_resumes_bit_for_bit(tmp_path, gpu_available):
         assert np.array_equal(dE_h[inner].ravel(order="F"), m.download("dE"))
         m.integrate(2)
         E5 = m.download("E")
-    r = subprocess.run([DRIVER, cf2, "E5.out", "0", "fp64", "restart"], capture_output=True, text=True,
+    r = subprocess.run([DRIVER, cf2, "out=E5.out", "restart"], capture_output=True, text=True,
                        timeout=120, cwd=tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "reading restart file" in r.stdout
@@ -139,7 +139,7 @@ def test_restart_files_cross_between_fortran_and_python(tmp_path, gpu_available)
         m.integrate(2)
         E5 = m.download("E")
     # Fortran writes, Python reads and resumes
-    r = subprocess.run([DRIVER, cf3, "E3.out", "0", "fp64", "restart"], capture_output=True, text=True,
+    r = subprocess.run([DRIVER, cf3, "out=E3.out", "restart"], capture_output=True, text=True,
                        timeout=120, cwd=tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
     f = tmp_path / restart.restart_file_name(0)
@@ -155,7 +155,7 @@ def test_restart_files_cross_between_fortran_and_python(tmp_path, gpu_available)
     assert got["itt"] == 5 and np.array_equal(got["E"], E5)
     # Python writes at step 3, Fortran reads and runs the last 2 steps
     restart.write_restart(str(f), case.n, 3, E3)
-    r = subprocess.run([DRIVER, cf2, "E5.out", "0", "fp64", "restart"], capture_output=True, text=True,
+    r = subprocess.run([DRIVER, cf2, "out=E5.out", "restart"], capture_output=True, text=True,
                        timeout=120, cwd=tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "reading restart file" in r.stdout
@@ -177,7 +177,7 @@ def test_driver_snapshots_equal_the_python_hosts(tmp_path, gpu_available):
     cases.write_case_file(case, cf, nsteps=7)
     fdir, pdir = tmp_path / "fortran", tmp_path / "python"
     fdir.mkdir()
-    r = subprocess.run([DRIVER, cf, "-", "3", "fp64"], capture_output=True, text=True, timeout=120, cwd=fdir)
+    r = subprocess.run([DRIVER, cf, "snap=3"], capture_output=True, text=True, timeout=120, cwd=fdir)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "time steps in file snap3D.cdf :      3" in r.stdout
     res = driver.run(case, nsteps=7, snapint=3 * case.dt * (1 + 1e-9), directory=str(pdir))
