@@ -224,6 +224,23 @@ int rt6d_download(rt6d_ctx *ctx, int field, int layout, double *out);
 int rt6d_diagnose(rt6d_ctx *ctx, double dt, rt6d_diag *out,
                   double *E_int, double *meanfl_int, double *diss6);
 
+/*
+ * rt6d_diagnose without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory -- code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc.  It
+ * runs rt6d_diagnose's kernels, with the same results bit for bit, on a
+ * file-scope device array of 4.5 MB per GPU (held until the process ends)
+ * instead of buffers of the context: the per-block partial sums, the reduced
+ * scalars and the staging of the 3D outputs.  Limits: with any 3D output
+ * requested, nx*ny*nz <= 65536 (RT6D_EINVAL beyond); the last step's dt must
+ * still be the context's coefficient dt (RT6D_ESTATE after an rt6d_fluxes
+ * with another dt).  All calls on one GPU share its array: calls that
+ * overlap on one GPU (other threads, other contexts) race; call it from one
+ * host thread per GPU at a time.
+ */
+int rt6d_diagnose_noalloc(rt6d_ctx *ctx, double dt, rt6d_diag *out,
+                          double *E_int, double *meanfl_int, double *diss6);
+
 /* Measurement helper with no reference counterpart: copies `bytes` between
  * two scratch device buffers with a plain streaming kernel `reps` times and
  * returns the average kernel time (ms).  Gives the achievable copy bandwidth

@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -15,6 +16,14 @@
 #include "rt6d_kernels.h"
 
 using namespace rt6d;
+
+// Scratch of rt6d_diagnose_noalloc, one copy per GPU in a file-scope device
+// array: the per-block partials of the diagnose sums, the reduced slots,
+// and the staging of the 3D outputs (E_int, meanfl_int, six diss fields)
+// for grids of up to kNoAllocMaxN3 (x,y,z) columns.  4.5 MB per GPU.
+constexpr size_t kNoAllocMaxN3 = 65536;
+constexpr size_t kNoAllocDoubles = (size_t)kDiagParts * kDiagSlots + 16 + 8 * kNoAllocMaxN3;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
 
 namespace {
 
@@ -50,6 +59,24 @@ Geo make_geo(const int *n)
     for (int d = 0; d < 5; ++d) g.div[d] = make_fastdiv((uint32_t)n[d]);
     g.ncells = (uint32_t)cells;
     return g;
+}
+
+// its device address on each GPU, resolved by rt6d_create with that device
+// current (the array's host-side name is not a device address)
+constexpr int kMaxDevices = 64;
+double *g_scratch[kMaxDevices] = {nullptr};
+std::mutex g_scratch_mu;
+
+int resolve_scratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return 0;   // rt6d_diagnose_noalloc then refuses
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[device]) return 0;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return fail(RT6D_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch[device] = (double *)p;
+    return 0;
 }
 
 }  // namespace
@@ -444,6 +471,8 @@ int rt6d_create(const int *n, int reflect_up, int reflect_down, int precision, r
         return bail(fail(RT6D_EHIP, "hipStreamCreate failed"));
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
         return bail(fail(RT6D_EHIP, "hipEventCreate failed"));
+    // set-up of rt6d_diagnose_noalloc: the scratch address on this device
+    if ((r = resolve_scratch(c->device))) return bail(r);
     size_t gtot = 0, ctot = 0;
     for (int d = 0; d < kAxes; ++d) {
         c->grid_off[d] = gtot;
@@ -950,43 +979,32 @@ int rt6d_download(rt6d_ctx *c, int field, int layout, double *out)
     return fail(RT6D_EINVAL, "rt6d_download: unknown field %d", field);
 }
 
-int rt6d_diagnose(rt6d_ctx *c, double dt, rt6d_diag *out, double *E_int, double *meanfl_int, double *diss6)
+// The diagnose kernels on buffers the caller provides: partial (nparts x
+// kDiagSlots doubles), dout (16 doubles) and, with 3D outputs, d3 (8 x n3
+// doubles); copies the results out and completes the reference's scalars.
+static int diag_run(rt6d_ctx *c, double dt, rt6d_diag *out, double *E_int, double *meanfl_int, double *diss6,
+                    double *partial, uint32_t nparts, double *dout, double *d3)
 {
-    int r;
-    if ((r = check_ctx(c))) return r;
-    if ((r = require_ready(c))) return r;
-    if (!out) return fail(RT6D_EINVAL, "rt6d_diagnose: null output");
-    if (!c->have_prev) return fail(RT6D_ESTATE, "rt6d_diagnose: no step has been taken since E was set");
-    const uint32_t nb = (uint32_t)((c->cells + 255) / 256);
-    if (c->nparts < nb) {
-        c->dfree((void *&)c->d_partial);
-        if ((r = c->dalloc((void **)&c->d_partial, (size_t)nb * kDiagSlots * sizeof(double)))) return r;
-        c->nparts = nb;
-    }
-    if (!c->d_dout && (r = c->dalloc((void **)&c->d_dout, 16 * sizeof(double)))) return r;
     const size_t n3 = (size_t)c->n[0] * c->n[1] * c->n[2];
     const bool want3 = E_int || meanfl_int || diss6;
-    if (want3 && !c->d_d3 && (r = c->dalloc((void **)&c->d_d3, 8 * n3 * sizeof(double)))) return r;
-    if ((r = ensure_coef(c, c->prev_dt))) return r;
     const void *Eprev = c->d_E[c->cur ^ 1];
     const void *Ecur = c->d_E[c->cur];
-    double *Ei = want3 ? c->d_d3 : nullptr, *Mi = want3 ? c->d_d3 + n3 : nullptr,
-           *Di = want3 ? c->d_d3 + 2 * n3 : nullptr;
+    double *Ei = want3 ? d3 : nullptr, *Mi = want3 ? d3 + n3 : nullptr, *Di = want3 ? d3 + 2 * n3 : nullptr;
     hipError_t e;
     if (c->prec == RT6D_FP64) {
         StepPtrs<double> p = step_ptrs<double>(c, Eprev, nullptr, c->prev_dt);
-        e = c->opt_exact ? launch_diag_exact(c->g, p, c->homog, (const double *)Ecur, c->d_partial, c->nparts,
-                                             c->d_dout, Ei, Mi, Di, c->stream)
-                         : launch_diag_fast<double>(c->g, p, c->homog, (const double *)Ecur, c->d_partial,
-                                                    c->nparts, c->d_dout, Ei, Mi, Di, c->stream);
+        e = c->opt_exact ? launch_diag_exact(c->g, p, c->homog, (const double *)Ecur, partial, nparts, dout, Ei, Mi,
+                                             Di, c->stream)
+                         : launch_diag_fast<double>(c->g, p, c->homog, (const double *)Ecur, partial, nparts, dout,
+                                                    Ei, Mi, Di, c->stream);
     } else {
         StepPtrs<float> p = step_ptrs<float>(c, Eprev, nullptr, c->prev_dt);
-        e = launch_diag_fast<float>(c->g, p, c->homog, (const float *)Ecur, c->d_partial, c->nparts, c->d_dout, Ei,
-                                    Mi, Di, c->stream);
+        e = launch_diag_fast<float>(c->g, p, c->homog, (const float *)Ecur, partial, nparts, dout, Ei, Mi, Di,
+                                    c->stream);
     }
     HIPCHK(e);
     double s[kDiagSlots];
-    HIPCHK(hipMemcpyAsync(s, c->d_dout, sizeof s, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(s, dout, sizeof s, hipMemcpyDeviceToHost, c->stream));
     if (want3) {
         if (E_int) HIPCHK(hipMemcpyAsync(E_int, Ei, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         if (meanfl_int) HIPCHK(hipMemcpyAsync(meanfl_int, Mi, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1011,6 +1029,48 @@ int rt6d_diagnose(rt6d_ctx *c, double dt, rt6d_diag *out, double *E_int, double 
     out->residual = out->eint + out->mflint + b[1] + b[3] + b[5] + b[7] + b[9] + b[11] - b[0] - b[2] - b[4] - b[6] -
                     b[8] - b[10];
     return 0;
+}
+
+int rt6d_diagnose(rt6d_ctx *c, double dt, rt6d_diag *out, double *E_int, double *meanfl_int, double *diss6)
+{
+    int r;
+    if ((r = check_ctx(c))) return r;
+    if ((r = require_ready(c))) return r;
+    if (!out) return fail(RT6D_EINVAL, "rt6d_diagnose: null output");
+    if (!c->have_prev) return fail(RT6D_ESTATE, "rt6d_diagnose: no step has been taken since E was set");
+    if (!c->d_partial) {
+        if ((r = c->dalloc((void **)&c->d_partial, (size_t)kDiagParts * kDiagSlots * sizeof(double)))) return r;
+        c->nparts = kDiagParts;
+    }
+    if (!c->d_dout && (r = c->dalloc((void **)&c->d_dout, 16 * sizeof(double)))) return r;
+    const size_t n3 = (size_t)c->n[0] * c->n[1] * c->n[2];
+    const bool want3 = E_int || meanfl_int || diss6;
+    if (want3 && !c->d_d3 && (r = c->dalloc((void **)&c->d_d3, 8 * n3 * sizeof(double)))) return r;
+    if ((r = ensure_coef(c, c->prev_dt))) return r;
+    return diag_run(c, dt, out, E_int, meanfl_int, diss6, c->d_partial, c->nparts, c->d_dout, c->d_d3);
+}
+
+int rt6d_diagnose_noalloc(rt6d_ctx *c, double dt, rt6d_diag *out, double *E_int, double *meanfl_int, double *diss6)
+{
+    int r;
+    if ((r = check_ctx(c))) return r;
+    if ((r = require_ready(c))) return r;
+    if (!out) return fail(RT6D_EINVAL, "rt6d_diagnose_noalloc: null output");
+    if (!c->have_prev) return fail(RT6D_ESTATE, "rt6d_diagnose_noalloc: no step has been taken since E was set");
+    const size_t n3 = (size_t)c->n[0] * c->n[1] * c->n[2];
+    if ((E_int || meanfl_int || diss6) && n3 > kNoAllocMaxN3)
+        return fail(RT6D_EINVAL,
+                    "rt6d_diagnose_noalloc: nx*ny*nz = %zu exceeds the %zu columns its scratch holds for the 3D "
+                    "outputs (pass NULL for them, or use rt6d_diagnose)", n3, kNoAllocMaxN3);
+    // the coefficients of the last step's dt are on the device from that step;
+    // uploading others would take a host buffer
+    if (c->coef_dt != c->prev_dt)
+        return fail(RT6D_ESTATE, "rt6d_diagnose_noalloc: the device coefficients are not those of the last step's dt "
+                    "(a later rt6d_fluxes changed them); step again or use rt6d_diagnose");
+    double *base = c->device < kMaxDevices ? g_scratch[c->device] : nullptr;
+    if (!base) return fail(RT6D_ESTATE, "rt6d_diagnose_noalloc: no scratch address for device %d", c->device);
+    double *partial = base, *dout = base + (size_t)kDiagParts * kDiagSlots, *d3 = dout + 16;
+    return diag_run(c, dt, out, E_int, meanfl_int, diss6, partial, kDiagParts, dout, d3);
 }
 
 int rt6d_copy_bandwidth(long long bytes, int reps, double *ms)

@@ -592,7 +592,7 @@ hipError_t launch_diag_exact(const Geo &g, const StepPtrs<double> &p, bool homog
                              double *partial, uint32_t nparts, double *out, double *Eint, double *mfint,
                              double *diss, hipStream_t s)
 {
-    const uint32_t nb = nblocks(g.ncells, 256);
+    const uint32_t nb = diag_blocks(g);
     if (nb > nparts) return hipErrorInvalidValue;
     if (homog)
         hipLaunchKernelGGL((k_diag_cells<double, true, true>), dim3(nb), dim3(256), 0, s, g, p, Ecur, partial);

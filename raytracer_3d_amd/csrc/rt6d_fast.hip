@@ -28,7 +28,7 @@ template <typename T>
 hipError_t launch_diag_fast(const Geo &g, const StepPtrs<T> &p, bool homog, const T *Ecur, double *partial,
                             uint32_t nparts, double *out, double *Eint, double *mfint, double *diss, hipStream_t s)
 {
-    const uint32_t nb = nblocks(g.ncells, 256);
+    const uint32_t nb = diag_blocks(g);
     if (nb > nparts) return hipErrorInvalidValue;
     if (homog)
         hipLaunchKernelGGL((k_diag_cells<T, false, true>), dim3(nb), dim3(256), 0, s, g, p, Ecur, partial);

@@ -215,16 +215,36 @@ __device__ __forceinline__ double wave_max(double v)
     return v;
 }
 
+// Blocks of k_diag_cells: a grid-stride loop over the cells in at most
+// kDiagParts blocks, so that the per-block partials take a bounded buffer.
+constexpr uint32_t kDiagParts = 2048;
+
+__host__ __device__ inline uint32_t diag_blocks(const Geo &g)
+{
+    const uint32_t nb = (g.ncells + 255u) / 256u;
+    return nb < kDiagParts ? (nb ? nb : 1u) : kDiagParts;
+}
+
+// a + x into the compensated sum (sum, cmp) (Neumaier)
+__device__ __forceinline__ void neumaier_add(double &sum, double &cmp, double x)
+{
+    const double t = sum + x;
+    cmp += fabs(sum) >= fabs(x) ? (sum - t) + x : (x - t) + sum;
+    sum = t;
+}
+
 template <typename T, bool EXACT, bool HOMOG>
 __global__ void __launch_bounds__(256) k_diag_cells(const Geo g, const StepPtrs<T> p, const T *__restrict__ Ecur,
                                                     double *__restrict__ partial)
 {
     __shared__ double red[4][kDiagSlots];
-    const uint32_t lin = blockIdx.x * blockDim.x + threadIdx.x;
-    double acc[kDiagSlots];
+    double acc[kDiagSlots], cmp[kDiagSlots];
 #pragma unroll
-    for (int s = 0; s < kDiagSlots; ++s) acc[s] = 0.0;
-    if (lin < g.ncells) {
+    for (int s = 0; s < kDiagSlots; ++s) acc[s] = cmp[s] = 0.0;
+    for (uint32_t lin = blockIdx.x * blockDim.x + threadIdx.x; lin < g.ncells; lin += gridDim.x * blockDim.x) {
+        double v[kDiagSlots];
+#pragma unroll
+        for (int s = 0; s < kDiagSlots; ++s) v[s] = 0.0;
         int c[kAxes];
         uint32_t q[kAxes + 1];
         decode(g, lin, c, q);
@@ -236,11 +256,11 @@ __global__ void __launch_bounds__(256) k_diag_cells(const Geo g, const StepPtrs<
 #pragma unroll
         for (int d = 0; d < kAxes; ++d) dxs[d] = p.ax.dx[d][c[d]];
         const double vol = dxs[0] * dxs[1] * dxs[2] * dxs[3] * dxs[4] * dxs[5];
-        acc[0] = (double)dE * vol;
-        acc[1] = (double)mf * vol;
+        v[0] = (double)dE * vol;
+        v[1] = (double)mf * vol;
         const double ec = (double)Ecur[lin];
         const double eps = (double)1e-12f;
-        acc[2] = fabs((double)mf) / (ec > eps ? ec : eps);
+        v[2] = fabs((double)mf) / (ec > eps ? ec : eps);
 #pragma unroll
         for (int d = 0; d < kAxes; ++d) {
             // product of the other five spacings in axis order
@@ -252,14 +272,19 @@ __global__ void __launch_bounds__(256) k_diag_cells(const Geo g, const StepPtrs<
                 w = first ? dxs[e] : w * dxs[e];
                 first = false;
             }
-            if (c[d] == 1) acc[3 + 2 * d] = (double)fl[d] * w;
-            if (c[d] == g.n[d]) acc[4 + 2 * d] = (double)fr[d] * w;
+            if (c[d] == 1) v[3 + 2 * d] = (double)fl[d] * w;
+            if (c[d] == g.n[d]) v[4 + 2 * d] = (double)fr[d] * w;
+        }
+#pragma unroll
+        for (int s = 0; s < kDiagSlots; ++s) {
+            if (s == 2) acc[s] = fmax(acc[s], v[s]);
+            else neumaier_add(acc[s], cmp[s], v[s]);
         }
     }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
     for (int s = 0; s < kDiagSlots; ++s) {
-        double v = s == 2 ? wave_max(acc[s]) : wave_sum(acc[s]);
+        double v = s == 2 ? wave_max(acc[s]) : wave_sum(acc[s] + cmp[s]);
         if (lane == 0) red[wv][s] = v;
     }
     __syncthreads();

@@ -41,7 +41,7 @@ EXPORTS = ("rt6d_version", "rt6d_last_error", "rt6d_device_count", "rt6d_create"
            "rt6d_ensemble_create", "rt6d_ensemble_destroy", "rt6d_ensemble_add", "rt6d_ensemble_remove",
            "rt6d_ensemble_step", "rt6d_ensemble_set_schedule", "rt6d_ensemble_tune", "rt6d_ensemble_info",
            "rt6d_init_separable", "rt6d_branch_flips", "rt6d_ensemble_step_async", "rt6d_ensemble_wait",
-           "rt6d_set_forcing")
+           "rt6d_set_forcing", "rt6d_diagnose_noalloc")
 
 
 class Diag(C.Structure):
@@ -94,6 +94,7 @@ def load_library(path: str | None = None):
         "rt6d_fluxes": ([vp, C.c_double], C.c_int),
         "rt6d_download": ([vp, C.c_int, C.c_int, dp], C.c_int),
         "rt6d_diagnose": ([vp, C.c_double, C.POINTER(Diag), dp, dp, dp], C.c_int),
+        "rt6d_diagnose_noalloc": ([vp, C.c_double, C.POINTER(Diag), dp, dp, dp], C.c_int),
         "rt6d_set_option": ([vp, C.c_int, C.c_int], C.c_int),
         "rt6d_query": ([vp, C.c_int, C.POINTER(C.c_longlong)], C.c_int),
         "rt6d_synchronize": ([vp], C.c_int),
@@ -281,16 +282,19 @@ class Model:
         dt = self.dt if dt is None else dt
         _check(self.lib, self.lib.rt6d_fluxes(self._h, dt), "rt6d_fluxes")
 
-    def diagnose(self, dt: float | None = None, fields3d: bool = False):
+    def diagnose(self, dt: float | None = None, fields3d: bool = False, noalloc: bool = False):
+        """rt6d_diagnose, or with noalloc rt6d_diagnose_noalloc (the same
+        kernels on the library's per-GPU scratch, no device allocation)."""
         dt = self.dt if dt is None else dt
         d = Diag()
+        name = "rt6d_diagnose_noalloc" if noalloc else "rt6d_diagnose"
+        fn = getattr(self.lib, name)
         if fields3d:
             n3 = self.n[0] * self.n[1] * self.n[2]
             Ei, Mi, Di = np.zeros(n3), np.zeros(n3), np.zeros(6 * n3)
-            _check(self.lib, self.lib.rt6d_diagnose(self._h, dt, C.byref(d), _dptr(Ei), _dptr(Mi), _dptr(Di)),
-                   "rt6d_diagnose")
+            _check(self.lib, fn(self._h, dt, C.byref(d), _dptr(Ei), _dptr(Mi), _dptr(Di)), name)
             return d.as_dict(), {"E_int": Ei, "meanfl_int": Mi, "diss": Di.reshape(6, n3)}
-        _check(self.lib, self.lib.rt6d_diagnose(self._h, dt, C.byref(d), None, None, None), "rt6d_diagnose")
+        _check(self.lib, fn(self._h, dt, C.byref(d), None, None, None), name)
         return d.as_dict()
 
     def download(self, field: str, layout: int | None = None) -> np.ndarray:
