@@ -63,14 +63,18 @@ extern "C" {
 #define RT6D_OPT_PATH 2      /* 0 auto (default), 1 force the general path
                                 (6D coefficient fields streamed from HBM) */
 #define RT6D_OPT_BLOCK 3     /* threads per block of the per-cell step kernel (64..1024) */
-#define RT6D_OPT_KERNEL 4    /* step kernel: 0 auto (default: the column kernel with two z
-                                cells per thread when nz is even, with two x cells per
-                                thread on the homogeneous path when nx is even; in
-                                binary32, where the grid gives each SIMD 2.5 wavefronts,
-                                with the wave-local neighbour exchange (option 22) when
-                                nx <= 24, else four x cells per thread (option 19) when
-                                nx is divisible by 4 and nz even; with
-                                the y neighbours staged in LDS on the general path);
+#define RT6D_OPT_KERNEL 4    /* step kernel: 0 auto (default: on the homogeneous path with
+                                nx even and <= 32, the pair kernel with the wave-local
+                                neighbour exchange, two z cells per thread (option 22)
+                                where nz is even and, in binary64, where that gives each
+                                SIMD 2.5 wavefronts, else one (option 21); binary32 grids
+                                that do not give each SIMD 2.5 wavefronts with one z cell
+                                per thread, and binary64 rows wider than 32, run the pair
+                                kernel (16/15); binary32 rows wider than 32 four x cells
+                                per thread (option 19) where nx is divisible by 4; odd nx
+                                the column kernel with two z cells per thread when nz is
+                                even; the general path the column kernel with the y
+                                neighbours staged in LDS);
                                 1 per-cell kernel;
                                 2/3 LDS marching kernel, 1/2 z cells per thread (nx*ny <= 1024);
                                 4/5/6 column kernel, 1/2/4 z cells per thread;

@@ -114,7 +114,7 @@ struct rt6d_ctx {
     bool params_ready = false;
     bool homog = false;
     int opt_exact = 0, opt_path = 0, opt_block = 256, opt_kernel = 0, opt_sync = 1;
-    void *d_hx = nullptr, *d_hy = nullptr, *d_hz = nullptr, *d_hm = nullptr, *d_hl = nullptr;
+    void *d_hx = nullptr, *d_hy = nullptr, *d_hz = nullptr, *d_hm = nullptr, *d_hl = nullptr, *d_hrec = nullptr;
     uint32_t n4 = 0, inner4m = 0;
     double vmax[kAxes] = {0};
     double cflsum_max = 0.0;
@@ -167,17 +167,20 @@ int ensure_coef(rt6d_ctx *c, double dt)
 {
     if (c->coef_dt == dt) return 0;
     size_t tot = 0;
-    for (int d = 0; d < kAxes; ++d) tot += 3 * (size_t)(c->n[d] + 1);
+    for (int d = 0; d < kAxes; ++d) tot += 5 * (size_t)(c->n[d] + 1);
     std::vector<double> h(tot);
     for (int d = 0; d < kAxes; ++d) {
         const int n = c->n[d];
         double *dx = h.data() + c->coef_off[d];
         double *rdx = dx + (n + 1);
         double *cf = rdx + (n + 1);
+        double *cr = cf + (n + 1);
         for (int f = 0; f <= n; ++f) {
             dx[f] = c->dt_h[d][f + 1];  // Fortran index f -> halo slot f+1
             rdx[f] = 1.0 / dx[f];
             cf[f] = dt / dx[f];
+            cr[2 * f] = cf[f];
+            cr[2 * f + 1] = rdx[f];
         }
     }
     HIPCHK(hipMemcpyAsync(c->d_coef, h.data(), tot * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -193,6 +196,7 @@ AxisCoef axis_coef(const rt6d_ctx *c)
         a.dx[d] = c->d_coef + c->coef_off[d];
         a.rdx[d] = a.dx[d] + (c->n[d] + 1);
         a.cfl[d] = a.rdx[d] + (c->n[d] + 1);
+        a.cr[d] = a.cfl[d] + (c->n[d] + 1);
     }
     return a;
 }
@@ -228,6 +232,7 @@ StepPtrs<T> step_ptrs(const rt6d_ctx *c, const void *Ein, void *Eout, double dt)
     p.hom.vz = (const T *)c->d_hz;
     p.hom.vm = (const T *)c->d_hm;
     p.hom.lam = (const T *)c->d_hl;
+    p.hom.rec = (const T *)c->d_hrec;
     p.hom.n4 = c->n4;
     p.hom.inner4m = c->inner4m;
     p.ax = axis_coef(c);
@@ -478,7 +483,7 @@ int rt6d_create(const int *n, int reflect_up, int reflect_down, int precision, r
         c->grid_off[d] = gtot;
         gtot += 2 * (size_t)(n[d] + 4);
         c->coef_off[d] = ctot;
-        ctot += 3 * (size_t)(n[d] + 1);
+        ctot += 5 * (size_t)(n[d] + 1);
         c->t_h[d].assign(n[d] + 4, 0.0);
         c->dt_h[d].assign(n[d] + 4, 0.0);
         size_t fc = cells / (uint64_t)n[d] * (uint64_t)(n[d] + 1);
@@ -525,6 +530,7 @@ int rt6d_destroy(rt6d_ctx *c)
     c->dfree(c->d_hz);
     c->dfree(c->d_hm);
     c->dfree(c->d_hl);
+    c->dfree(c->d_hrec);
     c->dfree(c->d_tdE);
     c->dfree(c->d_tmf);
     c->dfree((void *&)c->d_partial);
@@ -557,6 +563,12 @@ int rt6d_set_grid(rt6d_ctx *c, int axis, const double *t, const double *dt)
                           c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->grid_set[axis] = true;
+    if (axis < 2) {
+        // uniform spacing over the faces and cells the step kernels read (halo slots 1..n+1)
+        bool uni = true;
+        for (int p = 2; p <= c->n[axis] + 1; ++p) uni = uni && dt[p] == dt[1];
+        c->g.uxy = uni ? (c->g.uxy | (1 << axis)) : (c->g.uxy & ~(1 << axis));
+    }
     c->coef_dt = NAN;
     c->params_ready = false;
     return 0;
@@ -597,6 +609,7 @@ int rt6d_calc_parameter(rt6d_ctx *c)
     c->dfree(c->d_hz);
     c->dfree(c->d_hm);
     c->dfree(c->d_hl);
+    c->dfree(c->d_hrec);
     c->homog = false;
     c->params_ready = false;
     if ((r = ensure_coef(c, 1.0))) return r;
@@ -673,12 +686,24 @@ int rt6d_calc_parameter(rt6d_ctx *c)
         HIPCHK(hipMemcpyAsync(&bad, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         c->homog = bad == 0;
-        if (!c->homog) {
+        if (c->homog) {
+            if ((r = c->dalloc(&c->d_hrec, (size_t)kRec * c->n4 * es))) return r;
+            e = c->prec == RT6D_FP64
+                    ? launch_build_records<double>(c->g, (const double *)c->d_hx, (const double *)c->d_hy,
+                                                   (const double *)c->d_hz, (const double *)c->d_hm,
+                                                   (const double *)c->d_hl, (double *)c->d_hrec, c->stream)
+                    : launch_build_records<float>(c->g, (const float *)c->d_hx, (const float *)c->d_hy,
+                                                  (const float *)c->d_hz, (const float *)c->d_hm,
+                                                  (const float *)c->d_hl, (float *)c->d_hrec, c->stream);
+            HIPCHK(e);
+            HIPCHK(hipStreamSynchronize(c->stream));
+        } else {
             c->dfree(c->d_hx);
             c->dfree(c->d_hy);
             c->dfree(c->d_hz);
             c->dfree(c->d_hm);
             c->dfree(c->d_hl);
+    c->dfree(c->d_hrec);
         }
     }
     c->params_ready = true;

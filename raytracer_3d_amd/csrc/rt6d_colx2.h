@@ -84,9 +84,13 @@ struct ColX2Cfg {
 // reads above the writes.
 constexpr int kWLSlots = 128;
 
-template <typename T, bool EXACT, int ZC, bool FULL, bool WL = false>
+// UXY: the x and y spacings are uniform (every face and cell of both axes
+// has the same dx), so that their coefficients are wave-uniform scalars
+// rather than per-lane values (fast arithmetic only).
+template <typename T, bool EXACT, int ZC, bool FULL, bool WL = false, bool UXY = false>
 __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, const ColX2Cfg &cc, uint32_t b)
 {
+    static_assert(!(UXY && EXACT), "the exact arithmetic divides by the per-lane spacings");
     const uint32_t t = b * blockDim.x + threadIdx.x;
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2], nk = g.n[3], nl = g.n[4], nm = g.n[5];
     const int nxy = nx * ny, nxh = nx >> 1;
@@ -139,13 +143,22 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
     }
     // x faces x-1, x, x+1 and cells x, x+1; y faces y-1, y of row y
     double cx0 = 0, cx1 = 0, cx2 = 0, dx0 = 1, dx1 = 1, dx2 = 1, rxa = 0, rxb = 0;
-    if (ax) {
-        cx0 = p.ax.cfl[0][x - 1]; cx1 = p.ax.cfl[0][x]; cx2 = p.ax.cfl[0][x + 1];
-        dx0 = p.ax.dx[0][x - 1]; dx1 = p.ax.dx[0][x]; dx2 = p.ax.dx[0][x + 1];
-        rxa = p.ax.rdx[0][x]; rxb = p.ax.rdx[0][x + 1];
-    }
     double cyl = 0, cyr = 0, dyl = 1, dyr = 1, ryy = 0;
-    if (ay) { cyl = p.ax.cfl[1][y - 1]; cyr = p.ax.cfl[1][y]; dyl = p.ax.dx[1][y - 1]; dyr = p.ax.dx[1][y]; ryy = p.ax.rdx[1][y]; }
+    if constexpr (UXY) {
+        const double cx = sld(p.ax.cr[0], 2), rx = sld(p.ax.cr[0], 3);
+        const double cy = sld(p.ax.cr[1], 2), ry = sld(p.ax.cr[1], 3);
+        cx0 = cx1 = cx2 = cx;
+        rxa = rxb = rx;
+        cyl = cyr = cy;
+        ryy = ry;
+    } else {
+        if (ax) {
+            cx0 = p.ax.cfl[0][x - 1]; cx1 = p.ax.cfl[0][x]; cx2 = p.ax.cfl[0][x + 1];
+            dx0 = p.ax.dx[0][x - 1]; dx1 = p.ax.dx[0][x]; dx2 = p.ax.dx[0][x + 1];
+            rxa = p.ax.rdx[0][x]; rxb = p.ax.rdx[0][x + 1];
+        }
+        if (ay) { cyl = p.ax.cfl[1][y - 1]; cyr = p.ax.cfl[1][y]; dyl = p.ax.dx[1][y - 1]; dyr = p.ax.dx[1][y]; ryy = p.ax.rdx[1][y]; }
+    }
 
     int zsh[4];
     bool zmir[4];
@@ -204,6 +217,11 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
         FLm[j] = Pair<T>{T(0), T(0)};
     }
     Pair<T> zq[4];
+    auto ldhal = [&](int j, int mm) -> Pair<T> {
+        Pair<T> h{T(0), T(0)};
+        if (hval) h = bld2<T>(rE, ohal, (P0 + (uint32_t)j + (uint32_t)(mm - 1) * nzkl) * planeB);
+        return h;
+    };
     if (am) {
 #pragma unroll
         for (int j = 0; j < ZC; ++j) {
@@ -213,8 +231,10 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
     }
 
     for (int m = 1; m <= nm; ++m) {
-        const uint32_t q3 = Qkl + (uint32_t)(m - 1) * (uint32_t)nk * nl;
         const uint32_t q20 = P0 + (uint32_t)(m - 1) * nzkl;
+        // this step's coefficients: the plane records of the ZC cells, side by side
+        const T *rp = p.hom.rec + (size_t)kRec * q20;
+        auto rv = [&](int j, int f) -> T { return sld(rp, (uint32_t)(j * kRec + f)); };
         Pair<T> wnx[ZC];
 #pragma unroll
         for (int j = 0; j < ZC; ++j) {
@@ -226,10 +246,7 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
         Pair<T> hq[ZC];
         if constexpr (WL) {
 #pragma unroll
-            for (int j = 0; j < ZC; ++j) {
-                hq[j] = Pair<T>{T(0), T(0)};
-                if (hval) hq[j] = bld2<T>(rE, ohal, (P0 + (uint32_t)j + (uint32_t)(m - 1) * nzkl) * planeB);
-            }
+            for (int j = 0; j < ZC; ++j) hq[j] = ldhal(j, m);
         }
 
         T FzA[ZC + 1], FzB[ZC + 1];
@@ -242,7 +259,7 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
 #pragma unroll
             for (int i = 0; i <= ZC; ++i) {
                 const int fz = z0 - 1 + i;
-                const T v = sld(p.hom.vz, (uint32_t)fz + (uint32_t)(nz + 1) * q3);
+                const T v = i < ZC ? rv(i, kRecVzLo) : rv(ZC - 1, kRecVzHi);
                 const double dzf = sld(p.ax.dx[2], fz), czf = sld(p.ax.cfl[2], fz);
                 FzA[i] = face_flux<T, EXACT>(v, dtv, dzf, czf, za[i], za[i + 1], za[i + 2], za[i + 3]);
                 FzB[i] = face_flux<T, EXACT>(v, dtv, dzf, czf, zbv[i], zbv[i + 1], zbv[i + 2], zbv[i + 3]);
@@ -256,7 +273,6 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
             // (binary64 at 24^3 x 16^3: 0.396 instead of 0.417 ms)
             __builtin_amdgcn_s_setprio(1);
             const uint32_t pb = Pm * planeB;
-            const uint32_t q2 = q20 + (uint32_t)j;
             const Pair<T> e = w0[j];
             T dEa = T(0), dEb = T(0);
             [[maybe_unused]] V2 *sl = nullptr;
@@ -291,7 +307,7 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
                 }
                 if (xfirst) L = Pair<T>{e.a, e.a};
                 if (xlast) R = Pair<T>{e.b, e.b};
-                const T v0 = sld(p.hom.vx3, q2), vi = sld(p.hom.vx3, p.hom.n4 + q2), vb = sld(p.hom.vx3, 2 * p.hom.n4 + q2);
+                const T v0 = rv(j, kRecVx), vi = rv(j, kRecVx + 1), vb = rv(j, kRecVx + 2);
                 const T vL = x == 1 ? v0 : vi, vR = x + 1 == nx ? vb : vi;
                 const T F0 = face_flux<T, EXACT>(vL, dtv, dx0, cx0, L.a, L.b, e.a, e.b);
                 const T F1 = face_flux<T, EXACT>(vi, dtv, dx1, cx1, L.b, e.a, e.b, R.a);
@@ -313,7 +329,7 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
                     a4 = bld2<T>(rE, oy[3], pb);
                 }
                 __builtin_amdgcn_s_setprio(0);
-                const T v0 = sld(p.hom.vy3, q2), vi = sld(p.hom.vy3, p.hom.n4 + q2), vb = sld(p.hom.vy3, 2 * p.hom.n4 + q2);
+                const T v0 = rv(j, kRecVy), vi = rv(j, kRecVy + 1), vb = rv(j, kRecVy + 2);
                 const T vL = y == 1 ? v0 : vi, vR = y == ny ? vb : vi;
                 const T FLa = face_flux<T, EXACT>(vL, dtv, dyl, cyl, a0.a, a1.a, e.a, a3.a);
                 const T FRa = face_flux<T, EXACT>(vR, dtv, dyr, cyr, a1.a, e.a, a3.a, a4.a);
@@ -330,15 +346,16 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
                 dEb = div_term<T, EXACT>(dEb, FzB[j + 1], FzB[j], dzc, rzc);
             }
             if (am) {
-                const T vR = sld(p.hom.vm, q2 + p.hom.inner4m);
-                const double dm = sld(p.ax.dx[5], m), cm = sld(p.ax.cfl[5], m), rm = sld(p.ax.rdx[5], m);
+                const T vR = rv(j, kRecVmHi);
+                const double dm = EXACT ? sld(p.ax.dx[5], m) : 1.0;
+                const double cm = sld(p.ax.cr[5], 2 * m), rm = sld(p.ax.cr[5], 2 * m + 1);
                 const T FRa = face_flux<T, EXACT>(vR, dtv, dm, cm, wm1[j].a, e.a, wp1[j].a, wp2[j].a);
                 const T FRb = face_flux<T, EXACT>(vR, dtv, dm, cm, wm1[j].b, e.b, wp1[j].b, wp2[j].b);
                 dEa = div_term<T, EXACT>(dEa, FRa, FLm[j].a, dm, rm);
                 dEb = div_term<T, EXACT>(dEb, FRb, FLm[j].b, dm, rm);
                 FLm[j] = Pair<T>{FRa, FRb};
             }
-            const T lam = sld(p.hom.lam, q2);
+            const T lam = rv(j, kRecLam);
             dEa = dEa - e.a * lam;
             dEb = dEb - e.b * lam;
             Pair<T> en;
@@ -361,10 +378,10 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
     }
 }
 
-template <typename T, bool EXACT, int ZC, bool FULL, bool WL = false>
+template <typename T, bool EXACT, int ZC, bool FULL, bool WL = false, bool UXY = false>
 __global__ void __launch_bounds__(256) k_step_colx2(const Geo g, const StepPtrs<T> p, const ColX2Cfg cc, uint32_t nbl)
 {
     const uint32_t b = xcd_logical_block(blockIdx.x, nbl);
     if (b >= nbl) return;
-    colx2_body<T, EXACT, ZC, FULL, WL>(g, p, cc, b);
+    colx2_body<T, EXACT, ZC, FULL, WL, UXY>(g, p, cc, b);
 }

@@ -43,6 +43,7 @@ struct Geo {
     int act;             // bit d set when n[d] > 1 (the axis is advected)
     int up, down;        // z reflection switches
     uint32_t ncells;
+    int uxy;             // bit 0 (1): every x (y) spacing the step reads is the same
 };
 
 // Decode lin into 1-based coordinates c[d] and quotients q[d] = lin / inner[d].

@@ -335,6 +335,42 @@ hipError_t launch_build_homog(const Geo &g, const T *const vel[kAxes], const T *
                        vm, hlam, bad_dev);
     return hipGetLastError();
 }
+template <typename T>
+__global__ void __launch_bounds__(256) k_homog_records(const Geo g, const T *__restrict__ vx3,
+                                                       const T *__restrict__ vy3, const T *__restrict__ hz,
+                                                       const T *__restrict__ hm, const T *__restrict__ hlam,
+                                                       T *__restrict__ rec)
+{
+    const uint32_t nz = g.n[2];
+    const uint32_t n4 = g.ncells / ((uint32_t)g.n[0] * g.n[1]);
+    const uint32_t inner4m = nz * g.n[3] * g.n[4];
+    const uint32_t P = blockIdx.x * blockDim.x + threadIdx.x;
+    if (P >= n4) return;
+    const uint32_t z = P % nz, q = P / nz;   // z (0-based) and the (k,l,m) index
+    T *r = rec + (size_t)kRec * P;
+    for (int c = 0; c < 3; ++c) {
+        r[kRecVx + c] = vx3[c * n4 + P];
+        r[kRecVy + c] = vy3[c * n4 + P];
+    }
+    r[kRecVzLo] = hz[z + (nz + 1) * q];
+    r[kRecVzHi] = hz[z + 1 + (nz + 1) * q];
+    r[kRecVmHi] = hm[P + inner4m];
+    r[kRecLam] = hlam[P];
+}
+
+template <typename T>
+hipError_t launch_build_records(const Geo &g, const T *vx3, const T *vy3, const T *vz, const T *vm, const T *hlam,
+                                T *rec, hipStream_t s)
+{
+    const uint32_t n4 = g.ncells / ((uint32_t)g.n[0] * g.n[1]);
+    hipLaunchKernelGGL(k_homog_records<T>, dim3(nblocks(n4, 256)), dim3(256), 0, s, g, vx3, vy3, vz, vm, hlam, rec);
+    return hipGetLastError();
+}
+template hipError_t launch_build_records<double>(const Geo &, const double *, const double *, const double *,
+                                                 const double *, const double *, double *, hipStream_t);
+template hipError_t launch_build_records<float>(const Geo &, const float *, const float *, const float *,
+                                                const float *, const float *, float *, hipStream_t);
+
 template hipError_t launch_build_homog<double>(const Geo &, const double *const[kAxes], const double *, double *,
                                                double *, double *, double *, double *, int *, hipStream_t);
 template hipError_t launch_build_homog<float>(const Geo &, const float *const[kAxes], const float *, float *, float *,

@@ -14,6 +14,7 @@ struct AxisCoef {
     const double *dx[kAxes];
     const double *rdx[kAxes];
     const double *cfl[kAxes];
+    const double *cr[kAxes];   // (cfl[f], rdx[f]) interleaved: one wide scalar load per face
 };
 
 // Horizontally homogeneous coefficient tables over the 4D (z,k,l,m) space
@@ -22,9 +23,17 @@ struct AxisCoef {
 //   vz[(q2 + q3)]  faces along z: extent nz+1
 //   vm[q2 + f*inner4m] faces along m: extent nm+1
 //   lam[q2]
+//   rec[kRec*P ..]  per (z,k,l,m) plane P = lin / (nx*ny): the coefficients
+//                   one step of a cell of that plane reads, side by side, so
+//                   that a wave fetches them in one wide scalar load:
+//                   vx3 classes 0..2, vy3 classes 0..2, vz below and above,
+//                   vm above (face m), lam (the same values as the tables)
+constexpr int kRec = 10;
+enum RecField { kRecVx = 0, kRecVy = 3, kRecVzLo = 6, kRecVzHi = 7, kRecVmHi = 8, kRecLam = 9 };
 template <typename T>
 struct HomTables {
     const T *vx3, *vy3, *vz, *vm, *lam;
+    const T *rec;
     uint32_t n4, inner4m;
 };
 
@@ -593,6 +602,14 @@ hipError_t launch_colx2(const Geo &g, const StepPtrs<T> &p, hipStream_t stream)
     const uint64_t nthreads = (uint64_t)cc.nxyp * cc.nzb * g.n[3] * g.n[4];
     const uint32_t nbl = (uint32_t)((nthreads + 255) / 256);
     const uint32_t nbh = ((nbl + 7u) / 8u) * 8u;
+    // the wave-local kernels in fast arithmetic take uniform x and y spacings as scalars
+    if constexpr (WL && !EXACT) {
+        if ((g.uxy & 3) == 3 && is_full<true>(g)) {
+            hipLaunchKernelGGL((k_step_colx2<T, EXACT, ZC, true, WL, true>), dim3(nbh), dim3(256), 0, stream, g, p, cc,
+                               nbl);
+            return hipGetLastError();
+        }
+    }
     if (is_full<true>(g))
         hipLaunchKernelGGL((k_step_colx2<T, EXACT, ZC, true, WL>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
     else
@@ -665,15 +682,26 @@ inline int auto_kernel(const Geo &g, bool homog, int ncu)
         // 24^3 x 16^3 in binary64; the homogeneous path is faster without)
         return g.n[2] % 2 == 0 ? 10 : 9;
     }
+    // the pair kernel with the wave-local neighbour exchange where nx <= 32
+    // (24^3 x 16^3: binary64 0.300 against the plain pair kernel's 0.327 ms,
+    // binary32 0.142 against 0.195 for the quad kernel; 32^3 x 16^3: 0.652
+    // against 0.717 ms and 0.302 against 0.363 ms)
+    if (colx2wl_fits(g, true, 1)) {
+        if constexpr (sizeof(T) == 8) {
+            // two z cells per thread where that fills the device, else one:
+            // more wavefronts (12^3 x 8^3: 0.0159 against 0.0177 ms; 16^3 x
+            // 12^3: 0.047 against 0.054 ms)
+            return (g.n[2] % 2 == 0 && colx2_fills(g, 2, ncu)) ? 22 : 21;
+        } else {
+            // binary32 grids too small to fill the device with one z cell per
+            // thread run the plain pair kernel (12^3 x 8^3: 0.0132 against
+            // 0.0153 ms)
+            if (colx2_fills(g, 1, ncu)) return g.n[2] % 2 == 0 ? 22 : 21;
+        }
+    }
+    // binary32 rows too wide for the exchange: quads, two z cells per thread
     if constexpr (sizeof(T) == 4) {
-        // binary32, nx <= 24: pairs with the wave-local neighbour exchange
-        // (24^3 x 16^3: 0.204 against the quad kernel's 0.222 and the pair
-        // kernel's 0.231 ms)
-        if (g.n[0] <= 24 && colx2wl_fits(g, true, 2) && colx2_fills(g, 2, ncu)) return 22;
-        // binary32, wider rows: quads, two z cells per thread (32^3 x 16^3:
-        // 0.430 against the pair kernel's 0.497 and the wave-local pair
-        // kernel's 0.468 ms)
-        if (colq_fits(g, true, 2) && colq_fills(g, 2, ncu)) return 19;
+        if (g.n[0] > 32 && colq_fits(g, true, 2) && colq_fills(g, 2, ncu)) return 19;
     }
     // paired accesses, two z cells per thread where nz is even (binary64
     // 24^3 x 16^3: 10% faster than the column kernel)
@@ -832,6 +860,10 @@ hipError_t launch_param_omega(const ParamIn &pi, double *omega_i, double *omega_
 template <typename T>
 hipError_t launch_build_homog(const Geo &g, const T *const vel[kAxes], const T *lam, T *vx3, T *vy3, T *vz,
                               T *vm, T *hlam, int *bad_dev, hipStream_t s);
+// The plane records (HomTables::rec) from the homogeneous tables.
+template <typename T>
+hipError_t launch_build_records(const Geo &g, const T *vx3, const T *vy3, const T *vz, const T *vm, const T *hlam,
+                                T *rec, hipStream_t s);
 // max over faces 1..n_d of |v_d| into vmax_bits[6] (as ordered 64-bit ints)
 hipError_t launch_vmax(const Geo &g, const double *const vel[kAxes], unsigned long long *vmax_bits,
                        hipStream_t s);

@@ -96,12 +96,15 @@ def request_stream(rate: float, duration: float, seed: int = 4242, steps_range=(
 
 def resident_bytes(n, precision) -> int:
     """Device bytes of an integration after calc_parameter (E twice, the six
-    face-velocity fields, lambda, grids, coefficients, background)."""
+    face-velocity fields, lambda, the plane records of the homogeneous path,
+    grids, coefficients, background)."""
     cells = int(np.prod(n))
     es = 8 if precision == rt6d.FP64 else 4
     faces = sum(cells // n[d] * (n[d] + 1) for d in range(6))
     n3h = (n[0] + 4) * (n[1] + 4) * (n[2] + 4)
-    return es * (3 * cells + faces) + 8 * (3 * n3h) + 8 * 5 * sum(v + 4 for v in n) + (1 << 16)
+    planes = cells // (n[0] * n[1])
+    return (es * (3 * cells + faces) + es * 10 * planes + 8 * (3 * n3h) + 8 * 7 * sum(v + 4 for v in n)
+            + (1 << 16))
 
 
 def peak_bytes(n, precision) -> int:

@@ -241,14 +241,18 @@ def test_fp32_quad_kernel_against_reference(entry, kernel, gpu_available):
 
 
 @pytest.mark.parametrize("n, precision, path, expected", [
-    ((24, 24, 24, 16, 16, 16), rt6d.FP64, 0, 16),     # pairs, two z cells
+    ((24, 24, 24, 16, 16, 16), rt6d.FP64, 0, 22),     # wave-local pairs, two z cells
     ((24, 24, 24, 16, 16, 16), rt6d.FP32, 0, 22),     # wave-local pairs
-    ((32, 32, 32, 16, 16, 16), rt6d.FP32, 0, 19),     # quads
-    ((12, 12, 12, 8, 8, 8), rt6d.FP32, 0, 16),        # too small to fill: pairs
-    ((24, 24, 23, 16, 16, 16), rt6d.FP32, 0, 15),     # odd nz: pairs, one z cell
+    ((32, 32, 32, 16, 16, 16), rt6d.FP32, 0, 22),     # wave-local pairs up to nx = 32
+    ((12, 12, 12, 8, 8, 8), rt6d.FP32, 0, 16),        # too small to fill: plain pairs
+    ((12, 12, 12, 8, 8, 8), rt6d.FP64, 0, 21),        # too small for two z cells: one
+    ((24, 24, 23, 16, 16, 16), rt6d.FP32, 0, 21),     # odd nz: wave-local pairs, one z cell
+    ((36, 36, 36, 8, 8, 8), rt6d.FP32, 0, 19),        # nx > 32: quads
+    ((36, 36, 36, 8, 8, 8), rt6d.FP64, 0, 16),        # nx > 32: plain pairs
     ((23, 24, 24, 8, 8, 8), rt6d.FP64, 0, 5),         # odd nx: column kernel
     ((24, 24, 24, 8, 8, 8), rt6d.FP64, 1, 10),        # general path: LDS-staged y
-], ids=["fp64-24", "fp32-24", "fp32-32", "fp32-12", "fp32-oddz", "fp64-oddx", "general"])
+], ids=["fp64-24", "fp32-24", "fp32-32", "fp32-12", "fp64-12", "fp32-oddz", "fp32-36", "fp64-36", "fp64-oddx",
+        "general"])
 def test_auto_kernel_choice(n, precision, path, expected, gpu_available):
     """RT6D_Q_KERNEL reports what the auto choice resolves to for the grid
     (the choice is invisible in the results, so it is checked here)."""
