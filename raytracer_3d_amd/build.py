@@ -37,9 +37,11 @@ COMMON = [
     "-Wno-unused-variable",
 ]
 
+# -fno-honor-nans in the fast unit: the states are finite, so min/max need
+# not quiet signalling NaNs first (one instruction less per face limiter).
 UNITS = [
     ("rt6d_exact.hip", ["-ffp-contract=off"]),
-    ("rt6d_fast.hip", ["-ffp-contract=fast"]),
+    ("rt6d_fast.hip", ["-ffp-contract=fast", "-fno-honor-nans"]),
     ("rt6d_api.cpp", ["-x", "hip"]),
 ]
 

@@ -162,7 +162,28 @@ int check_ctx(rt6d_ctx *c)
     return 0;
 }
 
-// dx, rdx and dt/dx of every axis for step dt (binary64, on the device)
+AxisCoef axis_coef(const rt6d_ctx *c);
+
+// The plane records of the homogeneous path for the coefficients' dt (queued
+// on the context's stream).
+int build_records(rt6d_ctx *c)
+{
+    if (!c->d_hrec) return 0;
+    const AxisCoef ax = axis_coef(c);
+    const hipError_t e =
+        c->prec == RT6D_FP64
+            ? launch_build_records<double>(c->g, (const double *)c->d_hx, (const double *)c->d_hy,
+                                           (const double *)c->d_hz, (const double *)c->d_hm, (const double *)c->d_hl,
+                                           ax, (double *)c->d_hrec, c->stream)
+            : launch_build_records<float>(c->g, (const float *)c->d_hx, (const float *)c->d_hy,
+                                          (const float *)c->d_hz, (const float *)c->d_hm, (const float *)c->d_hl, ax,
+                                          (float *)c->d_hrec, c->stream);
+    HIPCHK(e);
+    return 0;
+}
+
+// dx, rdx and dt/dx of every axis for step dt (binary64, on the device), and
+// the plane records that depend on them
 int ensure_coef(rt6d_ctx *c, double dt)
 {
     if (c->coef_dt == dt) return 0;
@@ -184,6 +205,8 @@ int ensure_coef(rt6d_ctx *c, double dt)
         }
     }
     HIPCHK(hipMemcpyAsync(c->d_coef, h.data(), tot * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    int r;
+    if ((r = build_records(c))) return r;
     HIPCHK(hipStreamSynchronize(c->stream));
     c->coef_dt = dt;
     return 0;
@@ -688,14 +711,7 @@ int rt6d_calc_parameter(rt6d_ctx *c)
         c->homog = bad == 0;
         if (c->homog) {
             if ((r = c->dalloc(&c->d_hrec, (size_t)kRec * c->n4 * es))) return r;
-            e = c->prec == RT6D_FP64
-                    ? launch_build_records<double>(c->g, (const double *)c->d_hx, (const double *)c->d_hy,
-                                                   (const double *)c->d_hz, (const double *)c->d_hm,
-                                                   (const double *)c->d_hl, (double *)c->d_hrec, c->stream)
-                    : launch_build_records<float>(c->g, (const float *)c->d_hx, (const float *)c->d_hy,
-                                                  (const float *)c->d_hz, (const float *)c->d_hm,
-                                                  (const float *)c->d_hl, (float *)c->d_hrec, c->stream);
-            HIPCHK(e);
+            if ((r = build_records(c))) return r;
             HIPCHK(hipStreamSynchronize(c->stream));
         } else {
             c->dfree(c->d_hx);

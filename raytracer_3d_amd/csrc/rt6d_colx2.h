@@ -235,6 +235,20 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
         // this step's coefficients: the plane records of the ZC cells, side by side
         const T *rp = p.hom.rec + (size_t)kRec * q20;
         auto rv = [&](int j, int f) -> T { return sld(rp, (uint32_t)(j * kRec + f)); };
+        // a face coefficient set of cell j's record, and the velocity it came from
+        auto rk = [&](int j, int at, uint32_t bit) -> SbCoef<T> {
+            SbCoef<T> k;
+            k.vp = rv(j, at);
+            k.vn = rv(j, at + 1);
+            k.w = rv(j, at + 2);
+            const T fv = rv(j, kRecFlags);
+            uint32_t fl;
+            if constexpr (sizeof(T) == 8) fl = (uint32_t)__builtin_bit_cast(uint64_t, fv);
+            else fl = __builtin_bit_cast(uint32_t, fv);
+            k.pos = (fl & bit) != 0;
+            return k;
+        };
+        auto rvel = [&](int j, int at) -> T { return rv(j, at) + rv(j, at + 1); };   // vp + vn = v exactly
         Pair<T> wnx[ZC];
 #pragma unroll
         for (int j = 0; j < ZC; ++j) {
@@ -258,11 +272,17 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
             za[ZC + 2] = zq[2].a; za[ZC + 3] = zq[3].a; zbv[ZC + 2] = zq[2].b; zbv[ZC + 3] = zq[3].b;
 #pragma unroll
             for (int i = 0; i <= ZC; ++i) {
-                const int fz = z0 - 1 + i;
-                const T v = i < ZC ? rv(i, kRecVzLo) : rv(ZC - 1, kRecVzHi);
-                const double dzf = sld(p.ax.dx[2], fz), czf = sld(p.ax.cfl[2], fz);
-                FzA[i] = face_flux<T, EXACT>(v, dtv, dzf, czf, za[i], za[i + 1], za[i + 2], za[i + 3]);
-                FzB[i] = face_flux<T, EXACT>(v, dtv, dzf, czf, zbv[i], zbv[i + 1], zbv[i + 2], zbv[i + 3]);
+                if constexpr (EXACT) {
+                    const int fz = z0 - 1 + i;
+                    const T v = i < ZC ? rvel(i, kRecZlo) : rvel(ZC - 1, kRecZhi);
+                    const double dzf = sld(p.ax.dx[2], fz), czf = sld(p.ax.cfl[2], fz);
+                    FzA[i] = face_flux<T, EXACT>(v, dtv, dzf, czf, za[i], za[i + 1], za[i + 2], za[i + 3]);
+                    FzB[i] = face_flux<T, EXACT>(v, dtv, dzf, czf, zbv[i], zbv[i + 1], zbv[i + 2], zbv[i + 3]);
+                } else {
+                    const SbCoef<T> k = i < ZC ? rk(i, kRecZlo, kPosZlo) : rk(ZC - 1, kRecZhi, kPosZhi);
+                    FzA[i] = sb_flux_c<T>(k, za[i], za[i + 1], za[i + 2], za[i + 3]);
+                    FzB[i] = sb_flux_c<T>(k, zbv[i], zbv[i + 1], zbv[i + 2], zbv[i + 3]);
+                }
             }
         }
 #pragma unroll
@@ -307,11 +327,33 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
                 }
                 if (xfirst) L = Pair<T>{e.a, e.a};
                 if (xlast) R = Pair<T>{e.b, e.b};
-                const T v0 = rv(j, kRecVx), vi = rv(j, kRecVx + 1), vb = rv(j, kRecVx + 2);
-                const T vL = x == 1 ? v0 : vi, vR = x + 1 == nx ? vb : vi;
-                const T F0 = face_flux<T, EXACT>(vL, dtv, dx0, cx0, L.a, L.b, e.a, e.b);
-                const T F1 = face_flux<T, EXACT>(vi, dtv, dx1, cx1, L.b, e.a, e.b, R.a);
-                const T F2 = face_flux<T, EXACT>(vR, dtv, dx2, cx2, e.a, e.b, R.a, R.b);
+                const T v0 = rv(j, kRecVx0), vb = rv(j, kRecVxn);
+                T F0, F1, F2;
+                if constexpr (EXACT) {
+                    const T vi = rvel(j, kRecXi);
+                    const T vL = x == 1 ? v0 : vi, vR = x + 1 == nx ? vb : vi;
+                    F0 = face_flux<T, EXACT>(vL, dtv, dx0, cx0, L.a, L.b, e.a, e.b);
+                    F1 = face_flux<T, EXACT>(vi, dtv, dx1, cx1, L.b, e.a, e.b, R.a);
+                    F2 = face_flux<T, EXACT>(vR, dtv, dx2, cx2, e.a, e.b, R.a, R.b);
+                } else {
+                    // interior faces from the coefficient sets; the boundary
+                    // faces 0 and nx, whose zero-gradient ghost makes R = 0,
+                    // are v*E of the edge cell
+                    SbCoef<T> k0, k1, k2;
+                    if constexpr (UXY) {
+                        k0 = k1 = k2 = rk(j, kRecXi, kPosXi);
+                    } else {
+                        const T vi = rvel(j, kRecXi);
+                        k0 = sb_coef<T>(vi, (T)cx0);
+                        k1 = sb_coef<T>(vi, (T)cx1);
+                        k2 = sb_coef<T>(vi, (T)cx2);
+                    }
+                    F1 = sb_flux_c<T>(k1, L.b, e.a, e.b, R.a);
+                    F0 = sb_flux_c<T>(k0, L.a, L.b, e.a, e.b);
+                    F2 = sb_flux_c<T>(k2, e.a, e.b, R.a, R.b);
+                    if (xfirst) F0 = v0 * e.a;
+                    if (xlast) F2 = vb * e.b;
+                }
                 dEa = div_term<T, EXACT>(dEa, F1, F0, dx1, rxa);
                 dEb = div_term<T, EXACT>(dEb, F2, F1, dx2, rxb);
             }
@@ -329,12 +371,31 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
                     a4 = bld2<T>(rE, oy[3], pb);
                 }
                 __builtin_amdgcn_s_setprio(0);
-                const T v0 = rv(j, kRecVy), vi = rv(j, kRecVy + 1), vb = rv(j, kRecVy + 2);
-                const T vL = y == 1 ? v0 : vi, vR = y == ny ? vb : vi;
-                const T FLa = face_flux<T, EXACT>(vL, dtv, dyl, cyl, a0.a, a1.a, e.a, a3.a);
-                const T FRa = face_flux<T, EXACT>(vR, dtv, dyr, cyr, a1.a, e.a, a3.a, a4.a);
-                const T FLb = face_flux<T, EXACT>(vL, dtv, dyl, cyl, a0.b, a1.b, e.b, a3.b);
-                const T FRb = face_flux<T, EXACT>(vR, dtv, dyr, cyr, a1.b, e.b, a3.b, a4.b);
+                const T v0 = rv(j, kRecVy0), vb = rv(j, kRecVyn);
+                T FLa, FRa, FLb, FRb;
+                if constexpr (EXACT) {
+                    const T vi = rvel(j, kRecYi);
+                    const T vL = y == 1 ? v0 : vi, vR = y == ny ? vb : vi;
+                    FLa = face_flux<T, EXACT>(vL, dtv, dyl, cyl, a0.a, a1.a, e.a, a3.a);
+                    FRa = face_flux<T, EXACT>(vR, dtv, dyr, cyr, a1.a, e.a, a3.a, a4.a);
+                    FLb = face_flux<T, EXACT>(vL, dtv, dyl, cyl, a0.b, a1.b, e.b, a3.b);
+                    FRb = face_flux<T, EXACT>(vR, dtv, dyr, cyr, a1.b, e.b, a3.b, a4.b);
+                } else {
+                    SbCoef<T> kl, kr;
+                    if constexpr (UXY) {
+                        kl = kr = rk(j, kRecYi, kPosYi);
+                    } else {
+                        const T vi = rvel(j, kRecYi);
+                        kl = sb_coef<T>(vi, (T)cyl);
+                        kr = sb_coef<T>(vi, (T)cyr);
+                    }
+                    FLa = sb_flux_c<T>(kl, a0.a, a1.a, e.a, a3.a);
+                    FRa = sb_flux_c<T>(kr, a1.a, e.a, a3.a, a4.a);
+                    FLb = sb_flux_c<T>(kl, a0.b, a1.b, e.b, a3.b);
+                    FRb = sb_flux_c<T>(kr, a1.b, e.b, a3.b, a4.b);
+                    if (y == 1) { FLa = v0 * e.a; FLb = v0 * e.b; }
+                    if (y == ny) { FRa = vb * e.a; FRb = vb * e.b; }
+                }
                 dEa = div_term<T, EXACT>(dEa, FRa, FLa, dyr, ryy);
                 dEb = div_term<T, EXACT>(dEb, FRb, FLb, dyr, ryy);
             }
@@ -346,11 +407,19 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
                 dEb = div_term<T, EXACT>(dEb, FzB[j + 1], FzB[j], dzc, rzc);
             }
             if (am) {
-                const T vR = rv(j, kRecVmHi);
                 const double dm = EXACT ? sld(p.ax.dx[5], m) : 1.0;
-                const double cm = sld(p.ax.cr[5], 2 * m), rm = sld(p.ax.cr[5], 2 * m + 1);
-                const T FRa = face_flux<T, EXACT>(vR, dtv, dm, cm, wm1[j].a, e.a, wp1[j].a, wp2[j].a);
-                const T FRb = face_flux<T, EXACT>(vR, dtv, dm, cm, wm1[j].b, e.b, wp1[j].b, wp2[j].b);
+                const double rm = sld(p.ax.cr[5], 2 * m + 1);
+                T FRa, FRb;
+                if constexpr (EXACT) {
+                    const T vR = rvel(j, kRecMhi);
+                    const double cm = sld(p.ax.cr[5], 2 * m);
+                    FRa = face_flux<T, EXACT>(vR, dtv, dm, cm, wm1[j].a, e.a, wp1[j].a, wp2[j].a);
+                    FRb = face_flux<T, EXACT>(vR, dtv, dm, cm, wm1[j].b, e.b, wp1[j].b, wp2[j].b);
+                } else {
+                    const SbCoef<T> k = rk(j, kRecMhi, kPosMhi);
+                    FRa = sb_flux_c<T>(k, wm1[j].a, e.a, wp1[j].a, wp2[j].a);
+                    FRb = sb_flux_c<T>(k, wm1[j].b, e.b, wp1[j].b, wp2[j].b);
+                }
                 dEa = div_term<T, EXACT>(dEa, FRa, FLm[j].a, dm, rm);
                 dEb = div_term<T, EXACT>(dEb, FRb, FLm[j].b, dm, rm);
                 FLm[j] = Pair<T>{FRa, FRb};

@@ -132,17 +132,50 @@ __device__ __forceinline__ float flip_sign_by(float a, float b)
     return __builtin_bit_cast(float, ua ^ (ub & 0x80000000u));
 }
 
-// The same flux without a divide.  With R = E1-E0, Ru the upwind difference
-// and r = Ru/R, superbee's phi(r)*R = sign(R) * max(0, min(2*min(|R|,su),
-// max(|R|,su))) with su = Ru*sign(R): min/max/doubling only, so phi(r)*R is
-// exact (one of 0, R, Ru, 2R, 2Ru) and 0 for R == 0 as in the reference
-// (where the anti-diffusive term is multiplied by Rj = 0).
-// F = v/2*(E1+E0) - |v|/2*(R - (1-c)*phiR), c = |v|*dt/dx.
+// a*b + c with one rounding, in T
 template <typename T>
-__device__ __forceinline__ T sb_flux_fast(T v, T cfl_coef, T Em1, T E0, T E1, T E2)
+__device__ __forceinline__ T fma_t(T a, T b, T c)
+{
+    if constexpr (sizeof(T) == 4) return __builtin_fmaf(a, b, c);
+    else return __builtin_fma(a, b, c);
+}
+
+// The same flux without a divide, in upwind form.  With R = E1-E0, Ru the
+// upwind difference and r = Ru/R, superbee's phi(r)*R = sign(R) * max(0,
+// min(2*min(|R|,su), max(|R|,su))) with su = Ru*sign(R): min/max/doubling
+// only, so phi(r)*R is exact (one of 0, R, Ru, 2R, 2Ru) and 0 for R == 0 as
+// in the reference (where the anti-diffusive term is multiplied by Rj = 0).
+// The reference's v/2*(E1+E0) - |v|/2*R is v*E_up, E_up the upwind cell (E0
+// for v > 0, else E1), here fma(vp, E0, vn*E1) = v*E_up rounded once, with
+// vp = max(v,0) and vn = min(v,0), so that
+//   F = v*E_up + w*phiR,  w = |v|/2*(1-c),  c = |v|*dt/dx.
+// vp, vn, w and the upwind side depend on the face's velocity and spacing
+// only (sb_coef): kernels that read them from a table and kernels that form
+// them per face give the same bits.  On a boundary face with a zero-gradient
+// ghost, R = 0 and F = v*E_up exactly.
+template <typename T>
+struct SbCoef {
+    T vp, vn, w;
+    bool pos;   // v > 0: the upwind difference is E0 - Em1
+};
+
+template <typename T>
+__device__ __forceinline__ SbCoef<T> sb_coef(T v, T cfl_coef)
+{
+    const T a = fabs(v);
+    SbCoef<T> k;
+    k.vp = fmax(v, T(0));
+    k.vn = fmin(v, T(0));
+    k.w = (T(0.5) * a) * fma_t<T>(-a, cfl_coef, T(1));
+    k.pos = v > T(0);
+    return k;
+}
+
+template <typename T>
+__device__ __forceinline__ T sb_flux_c(const SbCoef<T> &k, T Em1, T E0, T E1, T E2)
 {
     const T R = E1 - E0;
-    const T Ru = v > T(0) ? (E0 - Em1) : (E2 - E1);
+    const T Ru = k.pos ? (E0 - Em1) : (E2 - E1);
     const T su = flip_sign_by(Ru, R);
     const T aR = fabs(R);
     const T mn = fmin(aR, su), mx = fmax(aR, su);
@@ -150,9 +183,13 @@ __device__ __forceinline__ T sb_flux_fast(T v, T cfl_coef, T Em1, T E0, T E1, T 
     if constexpr (sizeof(T) == 4) p = __builtin_amdgcn_fmed3f(mn + mn, T(0), mx);   // mx >= 0
     else p = fmax(fmin(mn + mn, mx), T(0));
     const T phiR = __builtin_copysign(p, R);
-    const T hv = T(0.5) * v;
-    const T c = fabs(v) * cfl_coef;
-    return hv * (E1 + E0) - fabs(hv) * (R - (T(1) - c) * phiR);
+    return fma_t<T>(k.w, phiR, fma_t<T>(k.vp, E0, k.vn * E1));
+}
+
+template <typename T>
+__device__ __forceinline__ T sb_flux_fast(T v, T cfl_coef, T Em1, T E0, T E1, T E2)
+{
+    return sb_flux_c<T>(sb_coef<T>(v, cfl_coef), Em1, E0, E1, E2);
 }
 
 }  // namespace rt6d

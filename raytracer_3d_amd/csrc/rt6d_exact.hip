@@ -339,7 +339,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_homog_records(const Geo g, const T *__restrict__ vx3,
                                                        const T *__restrict__ vy3, const T *__restrict__ hz,
                                                        const T *__restrict__ hm, const T *__restrict__ hlam,
-                                                       T *__restrict__ rec)
+                                                       const AxisCoef ax, T *__restrict__ rec)
 {
     const uint32_t nz = g.n[2];
     const uint32_t n4 = g.ncells / ((uint32_t)g.n[0] * g.n[1]);
@@ -347,29 +347,45 @@ __global__ void __launch_bounds__(256) k_homog_records(const Geo g, const T *__r
     const uint32_t P = blockIdx.x * blockDim.x + threadIdx.x;
     if (P >= n4) return;
     const uint32_t z = P % nz, q = P / nz;   // z (0-based) and the (k,l,m) index
+    const uint32_t m = P / inner4m;          // m (0-based)
     T *r = rec + (size_t)kRec * P;
-    for (int c = 0; c < 3; ++c) {
-        r[kRecVx + c] = vx3[c * n4 + P];
-        r[kRecVy + c] = vy3[c * n4 + P];
-    }
-    r[kRecVzLo] = hz[z + (nz + 1) * q];
-    r[kRecVzHi] = hz[z + 1 + (nz + 1) * q];
-    r[kRecVmHi] = hm[P + inner4m];
+    uint32_t flags = 0;
+    auto put = [&](int at, T v, double cfl, uint32_t bit) {
+        const SbCoef<T> k = sb_coef<T>(v, (T)cfl);
+        r[at] = k.vp;
+        r[at + 1] = k.vn;
+        r[at + 2] = k.w;
+        if (k.pos) flags |= bit;
+    };
+    put(kRecXi, vx3[1 * n4 + P], ax.cfl[0][1], kPosXi);
+    put(kRecYi, vy3[1 * n4 + P], ax.cfl[1][1], kPosYi);
+    put(kRecZlo, hz[z + (nz + 1) * q], ax.cfl[2][z], kPosZlo);
+    put(kRecZhi, hz[z + 1 + (nz + 1) * q], ax.cfl[2][z + 1], kPosZhi);
+    put(kRecMhi, hm[P + inner4m], ax.cfl[5][m + 1], kPosMhi);
+    r[kRecVx0] = vx3[0 * n4 + P];
+    r[kRecVxn] = vx3[2 * n4 + P];
+    r[kRecVy0] = vy3[0 * n4 + P];
+    r[kRecVyn] = vy3[2 * n4 + P];
     r[kRecLam] = hlam[P];
+    if constexpr (sizeof(T) == 8) r[kRecFlags] = __builtin_bit_cast(double, (uint64_t)flags);
+    else r[kRecFlags] = __builtin_bit_cast(float, flags);
+    for (int f = kRecFlags + 1; f < kRec; ++f) r[f] = T(0);
 }
 
 template <typename T>
 hipError_t launch_build_records(const Geo &g, const T *vx3, const T *vy3, const T *vz, const T *vm, const T *hlam,
-                                T *rec, hipStream_t s)
+                                const AxisCoef &ax, T *rec, hipStream_t s)
 {
     const uint32_t n4 = g.ncells / ((uint32_t)g.n[0] * g.n[1]);
-    hipLaunchKernelGGL(k_homog_records<T>, dim3(nblocks(n4, 256)), dim3(256), 0, s, g, vx3, vy3, vz, vm, hlam, rec);
+    hipLaunchKernelGGL(k_homog_records<T>, dim3(nblocks(n4, 256)), dim3(256), 0, s, g, vx3, vy3, vz, vm, hlam, ax,
+                       rec);
     return hipGetLastError();
 }
 template hipError_t launch_build_records<double>(const Geo &, const double *, const double *, const double *,
-                                                 const double *, const double *, double *, hipStream_t);
+                                                 const double *, const double *, const AxisCoef &, double *,
+                                                 hipStream_t);
 template hipError_t launch_build_records<float>(const Geo &, const float *, const float *, const float *,
-                                                const float *, const float *, float *, hipStream_t);
+                                                const float *, const float *, const AxisCoef &, float *, hipStream_t);
 
 template hipError_t launch_build_homog<double>(const Geo &, const double *const[kAxes], const double *, double *,
                                                double *, double *, double *, double *, int *, hipStream_t);

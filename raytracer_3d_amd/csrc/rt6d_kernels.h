@@ -23,13 +23,22 @@ struct AxisCoef {
 //   vz[(q2 + q3)]  faces along z: extent nz+1
 //   vm[q2 + f*inner4m] faces along m: extent nm+1
 //   lam[q2]
-//   rec[kRec*P ..]  per (z,k,l,m) plane P = lin / (nx*ny): the coefficients
-//                   one step of a cell of that plane reads, side by side, so
-//                   that a wave fetches them in one wide scalar load:
-//                   vx3 classes 0..2, vy3 classes 0..2, vz below and above,
-//                   vm above (face m), lam (the same values as the tables)
-constexpr int kRec = 10;
-enum RecField { kRecVx = 0, kRecVy = 3, kRecVzLo = 6, kRecVzHi = 7, kRecVmHi = 8, kRecLam = 9 };
+//   rec[kRec*P ..]  per (z,k,l,m) plane P = lin / (nx*ny): what one step of
+//                   a cell of that plane reads, side by side, so that a wave
+//                   fetches it in a few wide scalar loads: the fast-arithmetic
+//                   face coefficients (vp, vn, w of sb_coef, for the step's dt)
+//                   of the interior x and y faces (with the spacing of face
+//                   1, which is every face's where the spacing is uniform),
+//                   the z faces below and above and the m face above; the
+//                   velocities of the x and y boundary faces; lam; and bit
+//                   flags of which coefficient sets have v > 0
+constexpr int kRec = 24;
+enum RecField {
+    kRecXi = 0, kRecYi = 3, kRecZlo = 6, kRecZhi = 9, kRecMhi = 12,   // (vp, vn, w) each
+    kRecVx0 = 15, kRecVxn = 16, kRecVy0 = 17, kRecVyn = 18, kRecLam = 19, kRecFlags = 20
+};
+// flag bits of kRecFlags (in the low bits of that slot's bit pattern)
+enum RecFlag { kPosXi = 1, kPosYi = 2, kPosZlo = 4, kPosZhi = 8, kPosMhi = 16 };
 template <typename T>
 struct HomTables {
     const T *vx3, *vy3, *vz, *vm, *lam;
@@ -860,10 +869,11 @@ hipError_t launch_param_omega(const ParamIn &pi, double *omega_i, double *omega_
 template <typename T>
 hipError_t launch_build_homog(const Geo &g, const T *const vel[kAxes], const T *lam, T *vx3, T *vy3, T *vz,
                               T *vm, T *hlam, int *bad_dev, hipStream_t s);
-// The plane records (HomTables::rec) from the homogeneous tables.
+// The plane records (HomTables::rec) from the homogeneous tables and the
+// step coefficients ax (dt/dx for the step's dt).
 template <typename T>
 hipError_t launch_build_records(const Geo &g, const T *vx3, const T *vy3, const T *vz, const T *vm, const T *hlam,
-                                T *rec, hipStream_t s);
+                                const AxisCoef &ax, T *rec, hipStream_t s);
 // max over faces 1..n_d of |v_d| into vmax_bits[6] (as ordered 64-bit ints)
 hipError_t launch_vmax(const Geo &g, const double *const vel[kAxes], unsigned long long *vmax_bits,
                        hipStream_t s);

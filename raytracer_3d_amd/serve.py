@@ -103,7 +103,7 @@ def resident_bytes(n, precision) -> int:
     faces = sum(cells // n[d] * (n[d] + 1) for d in range(6))
     n3h = (n[0] + 4) * (n[1] + 4) * (n[2] + 4)
     planes = cells // (n[0] * n[1])
-    return (es * (3 * cells + faces) + es * 10 * planes + 8 * (3 * n3h) + 8 * 7 * sum(v + 4 for v in n)
+    return (es * (3 * cells + faces) + es * 24 * planes + 8 * (3 * n3h) + 8 * 7 * sum(v + 4 for v in n)
             + (1 << 16))
 
 
