@@ -110,6 +110,8 @@ extern "C" {
 #define RT6D_Q_CELLS 3
 #define RT6D_Q_STEPS 4       /* steps taken since the last upload of E */
 #define RT6D_Q_BYTES_PER_CELL 5 /* algorithmic HBM bytes per cell-update of the active step kernel */
+#define RT6D_Q_UNIFORM_XY 7  /* bit 0 (1): every x (y) spacing is the same, so the wave-local
+                                kernels take that axis's coefficients as scalars */
 #define RT6D_Q_KERNEL 6      /* the RT6D_OPT_KERNEL option rt6d_step runs: the set option, or
                                 what the auto choice resolves to (after calc_parameter, which
                                 decides the homogeneous path); an option that does not fit
@@ -288,11 +290,22 @@ int rt6d_ensemble_wait(rt6d_ensemble *ens, double *ms);
  * take tasks from per-XCD work counters.  order 0: the work list member by
  * member; 1: members interleaved block by block */
 int rt6d_ensemble_set_schedule(rt6d_ensemble *ens, int blocks_per_cu, int order);
-/* autotuner: times each schedule (blocks_per_cu 0, 2, 4, 8 x both orders) on
- * 2 + trial_steps real steps of the current members -- the members advance
- * by 8*(2 + trial_steps) steps; results do not depend on the schedule -- and
- * keeps the fastest */
-int rt6d_ensemble_tune(rt6d_ensemble *ens, int trial_steps, int *blocks_per_cu, int *order);
+/* tiles: the shape of the work a thread takes in each member (the results
+ * do not depend on it).  RT6D_TILES_WAVE_LOCAL (default): members with nx
+ * even and <= 32 run the pair kernel with the wave-local neighbour exchange,
+ * two z cells per thread where nz is even; RT6D_TILES_WAVE_LOCAL_1Z: the same
+ * with one z cell per thread; RT6D_TILES_PAIRS: no wave-local exchange
+ * (pairs, or quads in binary32).  Members the wave-local kernel does not fit
+ * take quads (binary32, nx divisible by 4), pairs (nx even) or columns. */
+#define RT6D_TILES_WAVE_LOCAL 0
+#define RT6D_TILES_WAVE_LOCAL_1Z 1
+#define RT6D_TILES_PAIRS 2
+int rt6d_ensemble_set_tiles(rt6d_ensemble *ens, int tiles);
+/* autotuner: times each tile choice x schedule (blocks_per_cu 0, 2, 4, 8 x
+ * both orders) on 2 + trial_steps real steps of the current members -- the
+ * members advance by 24*(2 + trial_steps) steps; results do not depend on
+ * tiles or schedule -- and keeps the fastest (any output may be NULL) */
+int rt6d_ensemble_tune(rt6d_ensemble *ens, int trial_steps, int *blocks_per_cu, int *order, int *tiles);
 /* number of members, total cells, steps taken by the ensemble */
 int rt6d_ensemble_info(rt6d_ensemble *ens, int *members, long long *cells, long long *steps);
 

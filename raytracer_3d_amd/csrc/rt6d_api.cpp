@@ -146,6 +146,12 @@ struct rt6d_ctx {
         if (p) hipFree(p);
         p = nullptr;
     }
+    // free a buffer dalloc counted with `bytes`
+    void dfree_n(void *&p, size_t bytes)
+    {
+        if (p) dev_bytes -= bytes;
+        dfree(p);
+    }
 };
 
 namespace {
@@ -625,14 +631,18 @@ int rt6d_calc_parameter(rt6d_ctx *c)
         if (!c->grid_set[d]) return fail(RT6D_ESTATE, "rt6d_calc_parameter: grid of axis %d not set", d);
     if (!c->bg_set) return fail(RT6D_ESTATE, "rt6d_calc_parameter: background not set");
     // release previous coefficient fields
-    for (int d = 0; d < kAxes; ++d) c->dfree(c->d_vel[d]);
-    c->dfree(c->d_lam);
-    c->dfree(c->d_hx);
-    c->dfree(c->d_hy);
-    c->dfree(c->d_hz);
-    c->dfree(c->d_hm);
-    c->dfree(c->d_hl);
-    c->dfree(c->d_hrec);
+    for (int d = 0; d < kAxes; ++d) c->dfree_n(c->d_vel[d], c->vel_count[d] * c->esz);
+    c->dfree_n(c->d_lam, c->cells * c->esz);
+    {
+        const size_t es = c->esz;
+        const size_t nzf = (size_t)c->n4 / c->n[2] * (c->n[2] + 1), nmf = (size_t)c->inner4m * (c->n[5] + 1);
+        c->dfree_n(c->d_hx, 3 * (size_t)c->n4 * es);
+        c->dfree_n(c->d_hy, 3 * (size_t)c->n4 * es);
+        c->dfree_n(c->d_hz, nzf * es);
+        c->dfree_n(c->d_hm, nmf * es);
+        c->dfree_n(c->d_hl, (size_t)c->n4 * es);
+        c->dfree_n(c->d_hrec, (size_t)kRec * c->n4 * es);
+    }
     c->homog = false;
     c->params_ready = false;
     if ((r = ensure_coef(c, 1.0))) return r;
@@ -713,13 +723,16 @@ int rt6d_calc_parameter(rt6d_ctx *c)
             if ((r = c->dalloc(&c->d_hrec, (size_t)kRec * c->n4 * es))) return r;
             if ((r = build_records(c))) return r;
             HIPCHK(hipStreamSynchronize(c->stream));
+            // no kernel of the homogeneous path reads the 6D coefficient
+            // fields: free them (downloads recompute them on demand)
+            for (int d = 0; d < kAxes; ++d) c->dfree_n(c->d_vel[d], c->vel_count[d] * es);
+            c->dfree_n(c->d_lam, c->cells * es);
         } else {
-            c->dfree(c->d_hx);
-            c->dfree(c->d_hy);
-            c->dfree(c->d_hz);
-            c->dfree(c->d_hm);
-            c->dfree(c->d_hl);
-    c->dfree(c->d_hrec);
+            c->dfree_n(c->d_hx, 3 * (size_t)c->n4 * es);
+            c->dfree_n(c->d_hy, 3 * (size_t)c->n4 * es);
+            c->dfree_n(c->d_hz, nzf * es);
+            c->dfree_n(c->d_hm, nmf * es);
+            c->dfree_n(c->d_hl, (size_t)c->n4 * es);
         }
     }
     c->params_ready = true;
@@ -865,11 +878,13 @@ static int do_steps(rt6d_ctx *c, double dt, int nsteps)
         void *in = c->d_E[c->cur], *out = c->d_E[c->cur ^ 1];
         hipError_t e;
         if (c->forcing) {   // set_forcing precedes integrate (main.f90:69-70)
+            // the homogeneous path keeps zdot only in its (z,k,l,m) table
+            const void *zd = c->homog ? c->d_hz : c->d_vel[2];
             e = c->prec == RT6D_FP64
-                    ? launch_bottom_forcing<double>(c->g, c->forcing, (double *)in, c->d_force,
-                                                    (const double *)c->d_vel[2], dt, dz1, c->stream)
-                    : launch_bottom_forcing<float>(c->g, c->forcing, (float *)in, c->d_force,
-                                                   (const float *)c->d_vel[2], dt, dz1, c->stream);
+                    ? launch_bottom_forcing<double>(c->g, c->forcing, (double *)in, c->d_force, (const double *)zd,
+                                                    c->homog, dt, dz1, c->stream)
+                    : launch_bottom_forcing<float>(c->g, c->forcing, (float *)in, c->d_force, (const float *)zd,
+                                                   c->homog, dt, dz1, c->stream);
             HIPCHK(e);
         }
         if (c->prec == RT6D_FP64) {
@@ -970,22 +985,37 @@ int rt6d_download(rt6d_ctx *c, int field, int layout, double *out)
         return fail(RT6D_EINVAL, "rt6d_download: layout %d for a flux", layout);
     }
     if (!c->params_ready) return fail(RT6D_ESTATE, "rt6d_download: calc_parameter has not run");
+    const ParamIn pi = param_in(c);
     if (field >= RT6D_FIELD_XDOT && field < RT6D_FIELD_XDOT + 6) {
         const int d = field - RT6D_FIELD_XDOT;
-        if (layout == RT6D_LAYOUT_FACES) return download_typed(c, c->d_vel[d], c->vel_count[d], out);
-        if (layout == RT6D_LAYOUT_INTERIOR) {
-            std::vector<double> tmp(c->vel_count[d]);
+        if (layout != RT6D_LAYOUT_FACES && layout != RT6D_LAYOUT_INTERIOR)
+            return fail(RT6D_EINVAL, "rt6d_download: layout %d for a velocity", layout);
+        std::vector<double> tmp(c->vel_count[d]);
+        if (c->d_vel[d]) {
             if ((r = download_typed(c, c->d_vel[d], c->vel_count[d], tmp.data()))) return r;
-            faces_to_interior(c->n, d, tmp.data(), out);
-            return 0;
+        } else {
+            // freed on the homogeneous path: recompute (binary64, rounded to
+            // the context's precision as the stored field was)
+            double *v = nullptr;
+            HIPCHK(hipMalloc((void **)&v, c->vel_count[d] * sizeof(double)));
+            hipError_t e = launch_param_velocity(d, pi, v, c->stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(tmp.data(), v, c->vel_count[d] * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+            hipFree(v);
+            HIPCHK(e);
+            if (c->prec == RT6D_FP32)
+                for (double &x : tmp) x = (double)(float)x;
         }
-        return fail(RT6D_EINVAL, "rt6d_download: layout %d for a velocity", layout);
+        if (layout == RT6D_LAYOUT_FACES) memcpy(out, tmp.data(), tmp.size() * sizeof(double));
+        else faces_to_interior(c->n, d, tmp.data(), out);
+        return 0;
     }
     if (layout != RT6D_LAYOUT_INTERIOR) return fail(RT6D_EINVAL, "rt6d_download: field %d is interior-only", field);
-    if (field == RT6D_FIELD_LAMBDA_SUM) return download_typed(c, c->d_lam, cells, out);
-    const ParamIn pi = param_in(c);
-    if (field >= RT6D_FIELD_LAMBDA_X && field < RT6D_FIELD_LAMBDA_X + 3) {
-        // recompute in binary64 from binary64 velocities
+    if (field == RT6D_FIELD_LAMBDA_SUM && c->d_lam) return download_typed(c, c->d_lam, cells, out);
+    if (field == RT6D_FIELD_LAMBDA_SUM || (field >= RT6D_FIELD_LAMBDA_X && field < RT6D_FIELD_LAMBDA_X + 3)) {
+        // recompute in binary64 from binary64 velocities (the sum rounded to
+        // the context's precision as the stored field was)
         double *v64[kAxes] = {nullptr}, *tmp = nullptr;
         double *lsum = nullptr;
         hipError_t e = hipSuccess;
@@ -996,10 +1026,11 @@ int rt6d_download(rt6d_ctx *c, int field, int layout, double *out)
         if (e == hipSuccess) e = hipMalloc((void **)&tmp, 3 * cells * sizeof(double));
         if (e == hipSuccess) e = hipMalloc((void **)&lsum, cells * sizeof(double));
         if (e == hipSuccess) e = launch_param_lambda(pi, v64, lsum, tmp, c->stream);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(out, tmp + (size_t)(field - RT6D_FIELD_LAMBDA_X) * cells, cells * sizeof(double),
-                               hipMemcpyDeviceToHost, c->stream);
+        const double *src = field == RT6D_FIELD_LAMBDA_SUM ? lsum : tmp + (size_t)(field - RT6D_FIELD_LAMBDA_X) * cells;
+        if (e == hipSuccess) e = hipMemcpyAsync(out, src, cells * sizeof(double), hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess && field == RT6D_FIELD_LAMBDA_SUM && c->prec == RT6D_FP32)
+            for (size_t i = 0; i < cells; ++i) out[i] = (double)(float)out[i];
         for (int d = 0; d < 3; ++d) if (v64[d]) hipFree(v64[d]);
         if (tmp) hipFree(tmp);
         if (lsum) hipFree(lsum);
@@ -1206,6 +1237,7 @@ int rt6d_query(rt6d_ctx *c, int what, long long *value)
     case RT6D_Q_DEVICE_BYTES: *value = (long long)c->dev_bytes; return 0;
     case RT6D_Q_CELLS: *value = (long long)c->cells; return 0;
     case RT6D_Q_STEPS: *value = c->steps; return 0;
+    case RT6D_Q_UNIFORM_XY: *value = c->g.uxy & 3; return 0;
     case RT6D_Q_KERNEL:
         *value = c->opt_kernel ? c->opt_kernel
                                : (c->prec == RT6D_FP64 ? auto_kernel<double>(c->g, c->homog, c->ncu)
@@ -1248,6 +1280,7 @@ struct rt6d_ensemble {
     std::vector<double> dt;
     std::vector<int> cur0;            // member's current buffer at the last rebuild
     int blocks_per_cu = 0, order = 0, ncu = 256;
+    int tiles = 0;                    // member tile shapes: RT6D_TILES_* of rt6d.h
     void *d_mem = nullptr;
     size_t mem_cap = 0;
     EnsTask *d_tasks = nullptr;
@@ -1288,15 +1321,18 @@ int ens_rebuild_t(rt6d_ensemble *e)
             M.p[0] = step_ptrs<T>(c, c->d_E[a], c->d_E[a ^ 1], e->dt[i]);
             M.p[1] = step_ptrs<T>(c, c->d_E[a ^ 1], c->d_E[a], e->dt[i]);
             const bool full = c->homog ? is_full<true>(c->g) : is_full<false>(c->g);
-            // the single-context kernel choice on a grid that fills the
-            // device (the members together do): in binary32 with two z cells
-            // per thread the wave-local pair kernel where nx <= 24, else
-            // quads where they fit; else pairs where nx is even
-            const bool wl = sizeof(T) == 4 && zc == 2 && c->g.n[0] <= 24 && colx2wl_fits(c->g, c->homog, 2);
+            // the member's tiles: the wave-local pair kernel where nx is even
+            // and <= 32 (two z cells per thread where nz is even, unless the
+            // ensemble's tiles ask for one), with uniform x/y spacings as
+            // scalars; else quads in binary32 where they fit; else pairs
+            // where nx is even, else the column kernel
+            const bool wl = e->tiles != RT6D_TILES_PAIRS && colx2wl_fits(c->g, c->homog, 1);
+            if (wl && e->tiles == RT6D_TILES_WAVE_LOCAL_1Z) zc = 1;
+            const bool uxy = wl && full && (c->g.uxy & 3) == 3;
             const bool q4 = !wl && sizeof(T) == 4 && zc == 2 && colq_fits(c->g, c->homog, 2);
             const bool x2 = !wl && !q4 && colx2_fits(c->g, c->homog, zc);
             v = (c->homog ? 1 : 0) | (zc == 2 ? 2 : 0) | (full ? 4 : 0) | (x2 ? 8 : 0) | (q4 ? 16 : 0) |
-                (wl ? 32 : 0);
+                (wl ? 32 : 0) | (uxy ? 64 : 0);
             M.variant = (uint32_t)v;
             e->cur0[i] = a;
             if (x2 || q4 || wl) {
@@ -1605,25 +1641,39 @@ int rt6d_ensemble_set_schedule(rt6d_ensemble *e, int blocks_per_cu, int order)
     return 0;
 }
 
-int rt6d_ensemble_tune(rt6d_ensemble *e, int trial_steps, int *bpc_out, int *order_out)
+int rt6d_ensemble_set_tiles(rt6d_ensemble *e, int tiles)
+{
+    if (!e || tiles < 0 || tiles > RT6D_TILES_PAIRS) return fail(RT6D_EINVAL, "rt6d_ensemble_set_tiles: bad arguments");
+    int r;
+    if ((r = ens_settle(e))) return r;
+    if (tiles != e->tiles) e->dirty = true;
+    e->tiles = tiles;
+    return 0;
+}
+
+int rt6d_ensemble_tune(rt6d_ensemble *e, int trial_steps, int *bpc_out, int *order_out, int *tiles_out)
 {
     if (!e || trial_steps < 2) return fail(RT6D_EINVAL, "rt6d_ensemble_tune: need >= 2 trial steps");
     const int bpcs[] = {0, 2, 4, 8};
     double best = 1e300;
-    int bb = e->blocks_per_cu, bo = e->order;
-    for (int order = 0; order <= 1; ++order)
-        for (int bpc : bpcs) {
-            int r = rt6d_ensemble_set_schedule(e, bpc, order);
-            if (r) return r;
-            double ms = 0.0;
-            if ((r = rt6d_ensemble_step(e, 2, nullptr))) return r;   // build + warm the graph
-            if ((r = rt6d_ensemble_step(e, trial_steps, &ms))) return r;
-            if (ms < best) { best = ms; bb = bpc; bo = order; }
-        }
-    int r = rt6d_ensemble_set_schedule(e, bb, bo);
+    int bb = e->blocks_per_cu, bo = e->order, bt = e->tiles;
+    for (int tiles = 0; tiles <= RT6D_TILES_PAIRS; ++tiles)
+        for (int order = 0; order <= 1; ++order)
+            for (int bpc : bpcs) {
+                int r = rt6d_ensemble_set_tiles(e, tiles);
+                if (!r) r = rt6d_ensemble_set_schedule(e, bpc, order);
+                if (r) return r;
+                double ms = 0.0;
+                if ((r = rt6d_ensemble_step(e, 2, nullptr))) return r;   // build + warm the graph
+                if ((r = rt6d_ensemble_step(e, trial_steps, &ms))) return r;
+                if (ms < best) { best = ms; bb = bpc; bo = order; bt = tiles; }
+            }
+    int r = rt6d_ensemble_set_tiles(e, bt);
+    if (!r) r = rt6d_ensemble_set_schedule(e, bb, bo);
     if (r) return r;
     if (bpc_out) *bpc_out = bb;
     if (order_out) *order_out = bo;
+    if (tiles_out) *tiles_out = bt;
     return 0;
 }
 

@@ -28,8 +28,9 @@ struct EnsMember {
                         // active; bit 3: two x cells per thread (homogeneous, nx even);
                         // bit 4: four x cells per thread (binary32, homogeneous, nx
                         // divisible by 4, two z cells per thread); bit 5: two x cells
-                        // per thread with the wave-local neighbour exchange (binary32,
-                        // homogeneous, nx even and <= 24, two z cells per thread)
+                        // per thread with the wave-local neighbour exchange (homogeneous,
+                        // nx even and <= 32); bit 6: with bit 5 and all axes active,
+                        // uniform x and y spacings taken as scalars
     uint32_t pad;
 };
 
@@ -37,7 +38,7 @@ struct EnsTask {
     uint32_t member, block;
 };
 
-constexpr int kEnsVariants = 64;
+constexpr int kEnsVariants = 128;
 constexpr int kEnsCounters = kEnsVariants * 8;   // one work counter per variant and XCD
 
 static __global__ void k_ens_reset(uint32_t *ctr)
@@ -45,11 +46,11 @@ static __global__ void k_ens_reset(uint32_t *ctr)
     if (threadIdx.x < kEnsCounters) ctr[threadIdx.x] = 0;
 }
 
-template <typename T, bool HOMOG, int ZC, bool FULL, bool X2, bool Q, bool W>
+template <typename T, bool HOMOG, int ZC, bool FULL, bool X2, bool Q, bool W, bool UXY>
 __device__ __forceinline__ void ens_task(const EnsMember<T> &M, uint32_t parity, uint32_t block)
 {
     if constexpr (W) {
-        colx2_body<T, false, ZC, FULL, true>(M.g, M.p[parity & 1u], M.cx, block);
+        colx2_body<T, false, ZC, FULL, true, UXY>(M.g, M.p[parity & 1u], M.cx, block);
     } else if constexpr (Q) {
         if constexpr (sizeof(T) == 4) colq_body<false, ZC, FULL>(M.g, M.p[parity & 1u], M.cx, block);
     } else if constexpr (X2) {
@@ -59,7 +60,7 @@ __device__ __forceinline__ void ens_task(const EnsMember<T> &M, uint32_t parity,
     }
 }
 
-template <typename T, bool HOMOG, int ZC, bool FULL, bool X2, bool Q, bool W, bool PERSIST>
+template <typename T, bool HOMOG, int ZC, bool FULL, bool X2, bool Q, bool W, bool UXY, bool PERSIST>
 __global__ void __launch_bounds__(256) k_ens_step(const EnsMember<T> *__restrict__ mem,
                                                   const EnsTask *__restrict__ tasks, uint32_t ntasks,
                                                   uint32_t parity, uint32_t *ctr)
@@ -68,7 +69,7 @@ __global__ void __launch_bounds__(256) k_ens_step(const EnsMember<T> *__restrict
         const uint32_t t = xcd_logical_block(blockIdx.x, ntasks);
         if (t >= ntasks) return;   // uniform over the block
         const EnsTask tk = tasks[t];
-        ens_task<T, HOMOG, ZC, FULL, X2, Q, W>(mem[tk.member], parity, tk.block);
+        ens_task<T, HOMOG, ZC, FULL, X2, Q, W, UXY>(mem[tk.member], parity, tk.block);
     } else {
         __shared__ uint32_t s_task;
         const uint32_t xcd = blockIdx.x & 7u;
@@ -82,7 +83,7 @@ __global__ void __launch_bounds__(256) k_ens_step(const EnsMember<T> *__restrict
             __syncthreads();   // every thread has read s_task before thread 0 writes it again
             if (t >= t1) break;
             const EnsTask tk = tasks[t];
-            ens_task<T, HOMOG, ZC, FULL, X2, Q, W>(mem[tk.member], parity, tk.block);
+            ens_task<T, HOMOG, ZC, FULL, X2, Q, W, UXY>(mem[tk.member], parity, tk.block);
         }
     }
 }

@@ -47,19 +47,20 @@ hipError_t launch_diag_fast(const Geo &g, const StepPtrs<T> &p, bool homog, cons
     return hipGetLastError();
 }
 
-template <typename T, bool HOMOG, int ZC, bool FULL, bool X2 = false, bool Q = false, bool W = false>
+template <typename T, bool HOMOG, int ZC, bool FULL, bool X2 = false, bool Q = false, bool W = false,
+          bool UXY = false>
 static void ens_launch_variant(const EnsMember<T> *mem, const EnsTask *tasks, uint32_t n, uint32_t parity,
                                uint32_t persist_blocks, uint32_t *ctr, hipStream_t s)
 {
     if (persist_blocks) {
         uint32_t nb = persist_blocks < n ? persist_blocks : n;
         nb = ((nb + 7u) / 8u) * 8u;
-        hipLaunchKernelGGL((k_ens_step<T, HOMOG, ZC, FULL, X2, Q, W, true>), dim3(nb), dim3(256), 0, s, mem, tasks, n, parity,
-                           ctr);
+        hipLaunchKernelGGL((k_ens_step<T, HOMOG, ZC, FULL, X2, Q, W, UXY, true>), dim3(nb), dim3(256), 0, s, mem, tasks,
+                           n, parity, ctr);
     } else {
         const uint32_t nb = ((n + 7u) / 8u) * 8u;
-        hipLaunchKernelGGL((k_ens_step<T, HOMOG, ZC, FULL, X2, Q, W, false>), dim3(nb), dim3(256), 0, s, mem, tasks, n,
-                           parity, ctr);
+        hipLaunchKernelGGL((k_ens_step<T, HOMOG, ZC, FULL, X2, Q, W, UXY, false>), dim3(nb), dim3(256), 0, s, mem,
+                           tasks, n, parity, ctr);
     }
 }
 
@@ -96,16 +97,17 @@ hipError_t launch_ensemble(const EnsMember<T> *mem, const EnsTask *tasks, const 
             if constexpr (sizeof(T) == 4)
                 ens_launch_variant<T, true, 2, true, false, true>(mem, tv, cnt[v], parity, persist_blocks, cv, s);
             break;
-        // wave-local pair kernel: binary32 homogeneous members with two z cells per thread
-        case 35:
-            if constexpr (sizeof(T) == 4)
-                ens_launch_variant<T, true, 2, false, false, false, true>(mem, tv, cnt[v], parity, persist_blocks, cv,
-                                                                         s);
+        // wave-local pair kernel: homogeneous members, one or two z cells per
+        // thread, with uniform x and y spacings as scalars (bit 6)
+        case 33: ens_launch_variant<T, true, 1, false, false, false, true>(mem, tv, cnt[v], parity, persist_blocks, cv, s); break;
+        case 35: ens_launch_variant<T, true, 2, false, false, false, true>(mem, tv, cnt[v], parity, persist_blocks, cv, s); break;
+        case 37: ens_launch_variant<T, true, 1, true, false, false, true>(mem, tv, cnt[v], parity, persist_blocks, cv, s); break;
+        case 39: ens_launch_variant<T, true, 2, true, false, false, true>(mem, tv, cnt[v], parity, persist_blocks, cv, s); break;
+        case 101:
+            ens_launch_variant<T, true, 1, true, false, false, true, true>(mem, tv, cnt[v], parity, persist_blocks, cv, s);
             break;
-        case 39:
-            if constexpr (sizeof(T) == 4)
-                ens_launch_variant<T, true, 2, true, false, false, true>(mem, tv, cnt[v], parity, persist_blocks, cv,
-                                                                        s);
+        case 103:
+            ens_launch_variant<T, true, 2, true, false, false, true, true>(mem, tv, cnt[v], parity, persist_blocks, cv, s);
             break;
         default: break;   // not produced by the host
         }

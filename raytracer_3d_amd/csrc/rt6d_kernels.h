@@ -882,8 +882,8 @@ hipError_t launch_cfl_max(const Geo &g, const double *const vel[kAxes], const Ax
                           unsigned long long *bits, hipStream_t s);
 hipError_t launch_convert_d2f(const double *in, float *out, size_t n, hipStream_t s);
 template <typename T>
-hipError_t launch_bottom_forcing(const Geo &g, int kind, T *E, const double *F, const T *zdot, double dt, double dz1,
-                                 hipStream_t s);
+hipError_t launch_bottom_forcing(const Geo &g, int kind, T *E, const double *F, const T *zdot, bool table, double dt,
+                                 double dz1, hipStream_t s);
 template <typename T>
 hipError_t launch_init_separable(const Geo &g, const double *f, double peak, double noise, uint64_t key, T *E,
                                  hipStream_t s);

@@ -31,7 +31,8 @@ FIELD = {"E": 0, "dE": 1, "meanfl": 2, "omega_i": 3, "omega_e": 4,
          "lambda_x": 17, "lambda_y": 18, "lambda_z": 19, "lambda": 20}
 LAYOUT_INTERIOR, LAYOUT_FACES, LAYOUT_HALO = 0, 1, 2
 OPT_EXACT, OPT_PATH, OPT_BLOCK, OPT_KERNEL, OPT_SYNC = 1, 2, 3, 4, 5
-Q_HOMOGENEOUS, Q_DEVICE_BYTES, Q_CELLS, Q_STEPS, Q_BYTES_PER_CELL, Q_KERNEL = 1, 2, 3, 4, 5, 6
+TILES_WAVE_LOCAL, TILES_WAVE_LOCAL_1Z, TILES_PAIRS = 0, 1, 2
+Q_HOMOGENEOUS, Q_DEVICE_BYTES, Q_CELLS, Q_STEPS, Q_BYTES_PER_CELL, Q_KERNEL, Q_UNIFORM_XY = 1, 2, 3, 4, 5, 6, 7
 
 # every symbol include/rt6d.h declares
 EXPORTS = ("rt6d_version", "rt6d_last_error", "rt6d_device_count", "rt6d_create", "rt6d_destroy",
@@ -41,7 +42,7 @@ EXPORTS = ("rt6d_version", "rt6d_last_error", "rt6d_device_count", "rt6d_create"
            "rt6d_ensemble_create", "rt6d_ensemble_destroy", "rt6d_ensemble_add", "rt6d_ensemble_remove",
            "rt6d_ensemble_step", "rt6d_ensemble_set_schedule", "rt6d_ensemble_tune", "rt6d_ensemble_info",
            "rt6d_init_separable", "rt6d_branch_flips", "rt6d_ensemble_step_async", "rt6d_ensemble_wait",
-           "rt6d_set_forcing", "rt6d_diagnose_noalloc")
+           "rt6d_set_forcing", "rt6d_diagnose_noalloc", "rt6d_ensemble_set_tiles")
 
 
 class Diag(C.Structure):
@@ -105,7 +106,8 @@ def load_library(path: str | None = None):
         "rt6d_ensemble_remove": ([vp, C.c_int], C.c_int),
         "rt6d_ensemble_step": ([vp, C.c_int, dp], C.c_int),
         "rt6d_ensemble_set_schedule": ([vp, C.c_int, C.c_int], C.c_int),
-        "rt6d_ensemble_tune": ([vp, C.c_int, ip, ip], C.c_int),
+        "rt6d_ensemble_tune": ([vp, C.c_int, ip, ip, ip], C.c_int),
+        "rt6d_ensemble_set_tiles": ([vp, C.c_int], C.c_int),
         "rt6d_ensemble_info": ([vp, ip, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)], C.c_int),
         "rt6d_init_separable": ([vp, dp, C.c_double, C.c_double, C.c_ulonglong, C.c_ulonglong], C.c_int),
         "rt6d_branch_flips": ([vp, vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)], C.c_int),
@@ -401,11 +403,16 @@ class Ensemble:
         _check(self.lib, self.lib.rt6d_ensemble_set_schedule(self._h, blocks_per_cu, order),
                "rt6d_ensemble_set_schedule")
 
+    def set_tiles(self, tiles: int):
+        """TILES_WAVE_LOCAL, TILES_WAVE_LOCAL_1Z or TILES_PAIRS (rt6d_ensemble_set_tiles)."""
+        _check(self.lib, self.lib.rt6d_ensemble_set_tiles(self._h, tiles), "rt6d_ensemble_set_tiles")
+
     def tune(self, trial_steps: int = 4):
-        b, o = C.c_int(), C.c_int()
-        _check(self.lib, self.lib.rt6d_ensemble_tune(self._h, trial_steps, C.byref(b), C.byref(o)),
+        """rt6d_ensemble_tune: returns (blocks_per_cu, order, tiles) of the fastest trial."""
+        b, o, t = C.c_int(), C.c_int(), C.c_int()
+        _check(self.lib, self.lib.rt6d_ensemble_tune(self._h, trial_steps, C.byref(b), C.byref(o), C.byref(t)),
                "rt6d_ensemble_tune")
-        return b.value, o.value
+        return b.value, o.value, t.value
 
     def info(self):
         m, c, s = C.c_int(), C.c_longlong(), C.c_longlong()

@@ -47,7 +47,7 @@ import numpy as np
 
 from . import cases, rt6d
 
-TUNE_STEPS = 8 * (2 + 2)   # rt6d_ensemble_tune with trial_steps = 2: 8 schedules x 4 steps
+TUNE_STEPS = 24 * (2 + 2)   # rt6d_ensemble_tune with trial_steps = 2: 3 tile choices x 8 schedules x 4 steps
 
 
 @dataclasses.dataclass
@@ -66,6 +66,7 @@ class Request:
     result: dict | None = None
     preemptions: int = 0
     resume: dict | None = None   # checkpoint of a preempted request: steps done / left, E or restart path
+    resident: int = 0            # device bytes charged while the member is resident
 
     @property
     def cells(self) -> int:
@@ -94,25 +95,38 @@ def request_stream(rate: float, duration: float, seed: int = 4242, steps_range=(
     return out
 
 
-def resident_bytes(n, precision) -> int:
-    """Device bytes of an integration after calc_parameter (E twice, the six
-    face-velocity fields, lambda, the plane records of the homogeneous path,
-    grids, coefficients, background)."""
+def _field_bytes(n, es):
+    """Bytes of the 6D coefficient fields (six face-velocity fields, lambda)
+    and of the homogeneous path's (z,k,l,m) tables and plane records."""
+    cells = int(np.prod(n))
+    faces = sum(cells // n[d] * (n[d] + 1) for d in range(6))
+    n4 = cells // (n[0] * n[1])
+    tables = 3 * n4 * 2 + n4 // n[2] * (n[2] + 1) + n4 // n[5] * (n[5] + 1) + n4
+    return es * (faces + cells), es * (tables + 24 * n4)
+
+
+def resident_bytes(n, precision, homogeneous=True) -> int:
+    """Device bytes of an integration after calc_parameter: E twice, grids,
+    coefficients, background, and either the homogeneous path's tables and
+    plane records (its 6D fields are freed) or the six face-velocity fields
+    and lambda."""
     cells = int(np.prod(n))
     es = 8 if precision == rt6d.FP64 else 4
-    faces = sum(cells // n[d] * (n[d] + 1) for d in range(6))
     n3h = (n[0] + 4) * (n[1] + 4) * (n[2] + 4)
-    planes = cells // (n[0] * n[1])
-    return (es * (3 * cells + faces) + es * 24 * planes + 8 * (3 * n3h) + 8 * 7 * sum(v + 4 for v in n)
-            + (1 << 16))
+    fields, tables = _field_bytes(n, es)
+    fixed = es * 2 * cells + 8 * (3 * n3h) + 8 * 7 * sum(v + 4 for v in n) + (1 << 16)
+    return fixed + (tables if homogeneous else fields)
 
 
 def peak_bytes(n, precision) -> int:
-    """Device bytes while calc_parameter runs (binary64 fields first)."""
+    """Device bytes while calc_parameter runs: the 6D fields (binary64 first,
+    then rounded for FP32) and the tables exist together."""
     cells = int(np.prod(n))
     faces = sum(cells // n[d] * (n[d] + 1) for d in range(6))
+    es = 8 if precision == rt6d.FP64 else 4
+    fields, tables = _field_bytes(n, es)
     extra = 0 if precision == rt6d.FP64 else 8 * (faces + cells)
-    return resident_bytes(n, precision) + extra
+    return resident_bytes(n, precision, False) + tables + extra
 
 
 def build_member(req: Request, cfl: float = 0.5):
@@ -225,8 +239,9 @@ class Server:
                     building -= 1
                     if err is not None:
                         raise err
+                    r.resident = resident_bytes(r.n, r.precision, bool(m.query(rt6d.Q_HOMOGENEOUS)))
                     with lock:
-                        self.used += resident_bytes(r.n, r.precision) - peak_bytes(r.n, r.precision)
+                        self.used += r.resident - peak_bytes(r.n, r.precision)
                     slot = self.ens[r.precision].add(m)
                     if not np.isfinite(r.started):
                         r.started = now()
@@ -277,7 +292,7 @@ class Server:
                         m.close()
                         r.done = now()
                         with lock:
-                            self.used -= resident_bytes(r.n, r.precision)
+                            self.used -= r.resident
                         done += 1
                         if log and done % 50 == 0:
                             log(f"served {done}/{total} at t={r.done:.1f}s, resident "
@@ -322,7 +337,7 @@ class Server:
         q.resume = resume
         m.close()
         with lock:
-            self.used -= resident_bytes(q.n, q.precision)
+            self.used -= q.resident
         q.preemptions += 1
         self.preemptions += 1
 
@@ -414,7 +429,7 @@ def ensemble_benchmark(members=96, steps=300, precision=rt6d.FP64, seed=4242, al
     with rt6d.Ensemble(precision) as ens:
         for m in ms:
             ens.add(m)
-        bpc, order = ens.tune(trial_steps=2)
+        bpc, order, tiles = ens.tune(trial_steps=2)
         ens.step(2)
         t1 = time.perf_counter()
         dev_ms = ens.step(steps)
@@ -430,7 +445,7 @@ def ensemble_benchmark(members=96, steps=300, precision=rt6d.FP64, seed=4242, al
         t_alone += m.integrate_timed(steps) * 1e-3
         w_alone += r.cells * steps
         m.close()
-    out = {"members": members, "steps": steps, "cells": cells, "setup_s": setup, "schedule": [bpc, order],
+    out = {"members": members, "steps": steps, "cells": cells, "setup_s": setup, "schedule": [bpc, order], "tiles": tiles,
            "wall_s": wall, "device_ms": dev_ms, "value": cells * steps / wall,
            "one_at_a_time": w_alone / t_alone if t_alone else None, "one_at_a_time_members": len(sample)}
     out["speedup_vs_one_at_a_time"] = out["value"] / out["one_at_a_time"] if out["one_at_a_time"] else None

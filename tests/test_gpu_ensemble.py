@@ -79,28 +79,37 @@ def test_ensemble_equals_individual_runs(precision, gpu_available):
 @pytest.mark.parametrize("precision", [rt6d.FP64, rt6d.FP32], ids=["fp64", "fp32"])
 def test_ensemble_member_kernel_variants(precision, gpu_available):
     """Members that take each ensemble kernel variant of the homogeneous path
-    -- in binary32 the wave-local pair kernel (nx <= 24), the quad kernel
-    (nx > 24, nx divisible by 4) and plain pairs (nx > 24, nx not divisible
-    by 4); in binary64 pairs with one and two z cells -- step together and
-    each equals its own run bit for bit, through the graph and the single
-    launches, in both block modes."""
+    under each tile choice -- the wave-local pair kernel (nx even and <= 32)
+    with two or one z cells per thread, with uniform x/y spacings as scalars
+    or per-lane spacings (non-uniform grid), and without the exchange the
+    plain pairs, quads in binary32 (nx divisible by 4) and the column kernel
+    (odd nx) -- step together and each equals its own run bit for bit,
+    through the graph and the single launches, in both block modes."""
     cs = [cases.packet_case(n=(8, 8, 4, 4, 4, 4), name="nx8"),
           cases.packet_case(n=(28, 4, 6, 4, 3, 4), name="nx28"),
           cases.packet_case(n=(26, 3, 4, 3, 3, 4), name="nx26"),
-          cases.packet_case(n=(32, 5, 5, 3, 4, 4), name="nx32_odd_nz")]
+          cases.packet_case(n=(32, 5, 5, 3, 4, 4), name="nx32_odd_nz"),
+          cases.packet_case(n=(36, 4, 4, 3, 3, 4), name="nx36"),
+          cases.packet_case(n=(7, 4, 4, 3, 3, 4), name="nx7"),
+          cases.random_case((8, 6, 6, 4, 4, 4), seed=24, horizontal=False, name="nonuniform_xy")]
     ms = [_model(c, precision) for c in cs]
     assert all(m.homogeneous for m in ms)
+    assert ms[-1].query(rt6d.Q_UNIFORM_XY) == 0 and ms[0].query(rt6d.Q_UNIFORM_XY) == 3
+    steps = 0
     with rt6d.Ensemble(precision) as ens:
         for m in ms:
             ens.add(m)
-        ens.set_schedule(0, 0)
-        ens.step(3)
-        ens.set_schedule(4, 1)
-        ens.step(4)
-        for c, m in zip(cs, ms):
-            E = m.download("E")
-            ref = _alone(c, precision, 7)
-            assert np.array_equal(E, ref), f"{c.name}: max diff {np.max(np.abs(E - ref))}"
+        for tiles in (rt6d.TILES_WAVE_LOCAL, rt6d.TILES_WAVE_LOCAL_1Z, rt6d.TILES_PAIRS):
+            ens.set_tiles(tiles)
+            ens.set_schedule(0, 0)
+            ens.step(3)
+            ens.set_schedule(4, 1)
+            ens.step(4)
+            steps += 7
+            for c, m in zip(cs, ms):
+                E = m.download("E")
+                ref = _alone(c, precision, steps)
+                assert np.array_equal(E, ref), f"{c.name}, tiles {tiles}: max diff {np.max(np.abs(E - ref))}"
     for m in ms:
         m.close()
 
@@ -111,10 +120,11 @@ def test_ensemble_tune_and_state(gpu_available):
     with rt6d.Ensemble(rt6d.FP64) as ens:
         for m in ms:
             ens.add(m)
-        bpc, order = ens.tune(trial_steps=2)
+        bpc, order, tiles = ens.tune(trial_steps=2)
         assert bpc in (0, 2, 4, 8) and order in (0, 1)
+        assert tiles in (rt6d.TILES_WAVE_LOCAL, rt6d.TILES_WAVE_LOCAL_1Z, rt6d.TILES_PAIRS)
         steps = ens.info()["steps"]
-        assert steps == 8 * (2 + 2)
+        assert steps == 24 * (2 + 2)
         ens.step(1)
         for c, m in zip(cs, ms):
             assert np.array_equal(m.download("E"), _alone(c, rt6d.FP64, steps + 1))

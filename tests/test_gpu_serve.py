@@ -86,9 +86,18 @@ def test_resident_estimate_covers_the_device_bytes(gpu_available):
     for n in SHAPES:
         for prec in (rt6d.FP64, rt6d.FP32):
             m = serve.build_member(serve.Request(0, 0.0, 3, n, 1, prec))
+            assert m.homogeneous
             got = m.query(rt6d.Q_DEVICE_BYTES)
             est = serve.resident_bytes(n, prec)
             m.close()
+            assert got <= est <= 1.5 * got + (1 << 16), (n, prec, got, est)
+            # a sheared background keeps the 6D fields
+            c = cases.random_case(n, seed=5)
+            with rt6d.Model.from_case(c, precision=prec) as m:
+                m.calc_parameter()
+                assert not m.homogeneous
+                got = m.query(rt6d.Q_DEVICE_BYTES)
+            est = serve.resident_bytes(n, prec, False)
             assert got <= est <= 1.5 * got + (1 << 16), (n, prec, got, est)
 
 
