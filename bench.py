@@ -8,12 +8,20 @@ tanh((z+500)/300) m/s, a Garrett-Munk-shaped spectrum with a seeded lognormal
 (for_src/integrate.f90:4-271) over the whole grid with E resident in HBM.
 
 Prints ONE JSON line (rank 0).  ``value`` is 6D cell-updates/s of the plain
-binary64 step as the library runs it on this input; ``roofline`` is that step
-kernel's algorithmic HBM bytes per launch over its event-timed average launch
-duration against the 8 TB/s MI355X peak; ``cpu_baseline`` times the reference
-Fortran (oracle/_ref, MPI ranks on the host cores) on a bounded sample of the
-same workload.  Extra keys report the FP32 form and the general (6D
-coefficient fields in HBM) path.
+binary64 step as the library runs it on this input.  ``roofline`` is that
+step kernel's algorithmic HBM bytes per launch over its average launch
+duration (HIP events on the library's stream around the K timed launches)
+against the 8 TB/s MI355X peak, and also against the copy ceiling measured
+in the same run (the library's full-grid 16-byte copy kernel); ``traffic`` is
+the HBM bytes per launch from the committed rocprofv3 counter passes.
+``cpu_baseline`` times the reference Fortran (oracle/_ref, MPI ranks on the
+host cores) on a bounded sample of the same workload.
+
+Extras (skip with --no-extras): the FP32 form and the general path, the
+32^3 x 16^3 grid, the FP32 error report of the 24^3 and 32^3 grids after 500
+steps, the Fortran drop-in driver on the same workload, the 96-member
+batched ensembles (every member also timed alone, same clock), and the
+serving scenario (Poisson 40 req/s for 120 s, 200 GB budget).
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -23,6 +31,7 @@ import argparse
 import json
 import math
 import os
+import re
 import subprocess
 import sys
 import tempfile
@@ -32,10 +41,25 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+SETTLE_SECONDS = 0.25
+COPY_BYTES = 1 << 30
 
 
 def _log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _settle(m, dt, seconds=SETTLE_SECONDS):
+    """Untimed steps for about `seconds` so that the GPU has left its idle
+    clocks before the warm-up (a production run steps for minutes); returns
+    the number of steps taken."""
+    t0 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t0 < seconds:
+        m.integrate(10, dt=dt)
+        m.synchronize()
+        n += 10
+    return n
 
 
 def _time_steps(m, dt, steps, warmup):
@@ -116,32 +140,56 @@ def cpu_baseline(case, dt, steps, ranks=(16, 8, 4, 2), label="24^3x16^3"):
 
 
 def _traffic(workload, tag):
-    """HBM bytes per launch of the step kernel from the committed rocprofv3
-    --pmc summary (FETCH_SIZE and WRITE_SIZE, scaled by the copy kernel)."""
+    """The committed rocprofv3 counter figures of the step kernel
+    (profiles/traffic.json): HBM bytes per launch from the EA requests to
+    DRAM and from FETCH_SIZE + WRITE_SIZE, each scaled by the copy kernel."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(p):
         return None
     try:
         with open(p) as f:
-            t = json.load(f)
-        e = t.get(workload, {}).get(tag)
-        return e["bytes_per_launch"] if e else None
+            return json.load(f).get(workload, {}).get(tag)
     except Exception:  # noqa: BLE001
         return None
 
 
-def _step_kernel(precision, option):
+def _step_kernel(precision, option, uxy):
     """Name of the kernel one step launches for a resolved kernel option
     (rt6d_query RT6D_Q_KERNEL), compiled for all advected axes active."""
     T = precision
+    wl = f"k_step_colx2<{T}, false, {{}}, true, true, {'true' if uxy else 'false'}>"
     names = {
-        15: f"k_step_colx2<{T}, false, 1, true>", 16: f"k_step_colx2<{T}, false, 2, true>",
-        21: f"k_step_colx2<{T}, false, 1, true, true>", 22: f"k_step_colx2<{T}, false, 2, true, true>",
-        18: "k_step_colq<false, 1, true>", 19: "k_step_colq<false, 2, true>",
+        15: f"k_step_colx2<{T}, false, 1, true, false, false>", 16: f"k_step_colx2<{T}, false, 2, true, false, false>",
+        21: wl.format(1), 22: wl.format(2),
+        18: "k_step_colq<false, 1, true, false>", 19: "k_step_colq<false, 2, true, false>",
         9: f"k_step_col<{T}, false, false, 1, true, 1>", 10: f"k_step_col<{T}, false, false, 2, true, 1>",
         4: f"k_step_col<{T}, false, true, 1, true, 0>", 5: f"k_step_col<{T}, false, true, 2, true, 0>",
     }
     return names.get(option, f"kernel option {option}")
+
+
+def fortran_dropin(case, dt, steps, precision="fp64"):
+    """The Fortran driver (fortran/rt6d_driver: the reference's set-up order
+    and time loop over the drop-in integrate) on the same workload; its own
+    system_clock around the loop."""
+    drv = os.path.join(ROOT, "raytracer_3d_amd", "fortran", "rt6d_driver")
+    if not os.path.exists(drv):
+        return None
+    from raytracer_3d_amd import cases
+    with tempfile.TemporaryDirectory() as tmp:
+        cf = os.path.join(tmp, "bench.case")
+        cases.write_case_file(case, cf, nsteps=steps, dt=dt)
+        cmd = [drv, cf] + (["fp32"] if precision == "fp32" else [])
+        p = subprocess.run(cmd, cwd=tmp, capture_output=True, text=True, timeout=600)
+    if p.returncode != 0:
+        return {"error": (p.stdout + p.stderr)[-400:]}
+    mt = re.search(r"main loop\s+([0-9.Ee+-]+)\s*s,\s*([0-9.Ee+-]+)\s*cell-updates/s", p.stdout)
+    if not mt:
+        return {"error": p.stdout[-400:]}
+    secs = float(mt.group(1))
+    return {"value": float(mt.group(2)), "seconds": secs, "steps": steps, "ms_per_step": 1e3 * secs / steps,
+            "how": "rt6d_driver CASE: set_parameter .. calc_parameter, then int(runlen/dt) integrate calls "
+                   "(main.f90:51-79), system_clock around the loop and the final synchronize"}
 
 
 def main():
@@ -153,8 +201,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=300)
-    ap.add_argument("--serve-seconds", type=float, default=10.0,
-                    help="arrival window of the serving extra (the full scenario is 120 s; 0 skips it)")
+    ap.add_argument("--serve-seconds", type=float, default=120.0,
+                    help="arrival window of the serving extra (the scenario is 120 s; 0 skips it)")
+    ap.add_argument("--error-steps", type=int, default=500, help="steps of the FP32 error report (0 skips it)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -175,6 +224,7 @@ def main():
         m = rt6d.Model.from_case(c, precision=precision, path=path, kernel=kernel)
         m.calc_parameter()
         dt = m.cfl_dt(0.5)
+        settle = _settle(m, dt)
         wall, ev_ms = _time_steps(m, dt, args.steps, args.warmup)
         res = {
             "value": c.cells * args.steps / wall,
@@ -183,24 +233,38 @@ def main():
             "bytes_per_cell": m.bytes_per_cell,
             "homogeneous": m.homogeneous,
             "kernel_option": m.kernel,
+            "uniform_xy": m.query(rt6d.Q_UNIFORM_XY),
             "dt": dt,
+            "settle_steps": settle,
         }
         res["achieved_GBs"] = res["bytes_per_cell"] * c.cells / (res["launch_ms"] * 1e-3) / 1e9
         m.close()
         return res
 
+    _log("copy ceiling: 1 GiB device copy, 16 B per thread")
+    copy_ms = rt6d.device_copy_timed(COPY_BYTES, reps=20)
+    copy_gbs = 2 * COPY_BYTES / (copy_ms * 1e-3) / 1e9
+
     _log("binary64 step")
     main_res = run(rt6d.FP64, 0)
     extras = {}
     if not args.no_extras:
-        _log("binary64 step, general path")
-        extras["fp64_general"] = run(rt6d.FP64, 1)
         _log("FP32 form")
         extras["fp32"] = run(rt6d.FP32, 0)
         extras["fp32"]["speedup_vs_fp64"] = extras["fp32"]["value"] / main_res["value"]
+        _log("binary64 and FP32 steps, general path (6D coefficient fields in HBM)")
+        extras["fp64_general"] = run(rt6d.FP64, 1)
         extras["fp32_general"] = run(rt6d.FP32, 1)
         _log("binary64 step, per-cell kernel")
         extras["fp64_percell"] = run(rt6d.FP64, 0, kernel=1)
+        _log("Fortran drop-in driver, 500 steps of the same workload")
+        extras["fortran_dropin_fp64"] = fortran_dropin(case, main_res["dt"], 500)
+        extras["fortran_dropin_fp32"] = fortran_dropin(case, main_res["dt"], 500, "fp32")
+        if args.error_steps > 0:
+            from raytracer_3d_amd import precision
+            _log(f"FP32 error report, 24^3 x 16^3, {args.error_steps} steps")
+            extras["fp32_error_24cube"] = precision.fp32_report(
+                case, steps=args.error_steps, checkpoints=[1, 100, args.error_steps], log=_log)
         _log("32^3 x 16^3 (134M cells): binary64 step and FP32 form")
         big = cases.gm_case(n=(32, 32, 32, 16, 16, 16))
         extras["fp64_32cube"] = run(rt6d.FP64, 0, c=big)
@@ -208,6 +272,10 @@ def main():
         extras["fp32_32cube"]["speedup_vs_fp64"] = extras["fp32_32cube"]["value"] / extras["fp64_32cube"]["value"]
         for k in ("fp64_32cube", "fp32_32cube"):
             extras[k]["cells"] = big.cells
+        if args.error_steps > 0:
+            _log(f"FP32 error report, 32^3 x 16^3, {args.error_steps} steps")
+            extras["fp32_error_32cube"] = precision.fp32_report(
+                big, steps=args.error_steps, checkpoints=[1, 100, args.error_steps], end_fields=False, log=_log)
         del big
         _log("config-0 packet 32x4x32x16x4x16 (4.2M cells): binary64 step, reference Fortran on 4 ranks")
         c0 = cases.packet_case()
@@ -218,26 +286,35 @@ def main():
                 c0, extras["config0_packet_fp64"]["dt"], 20, ranks=(4, 2), label="config-0 packet 32x4x32x16x4x16")
         from raytracer_3d_amd import serve
         for prec, tag in ((rt6d.FP64, "ensemble96_fp64"), (rt6d.FP32, "ensemble96_fp32")):
-            _log(f"batched ensemble, 96 members x {args.ensemble_steps} steps ({tag[-4:]})")
-            extras[tag] = serve.ensemble_benchmark(96, args.ensemble_steps, prec, alone_members=24, log=_log)
+            _log(f"batched ensemble, 96 members x {args.ensemble_steps} steps ({tag[-4:]}), every member also alone")
+            extras[tag] = serve.ensemble_benchmark(96, args.ensemble_steps, prec, log=_log)
         if args.serve_seconds > 0:
             _log(f"serving: Poisson 40 req/s for {args.serve_seconds:g} s, 200 GB budget")
             extras["serving"] = serve.serve_benchmark(40.0, args.serve_seconds, 200.0, baseline_seconds=10.0,
                                                       log=_log)
 
-    kname = _step_kernel("double", main_res["kernel_option"])
+    kname = _step_kernel("double", main_res["kernel_option"], main_res["uniform_xy"] == 3)
     ttag = "fp64_auto" if main_res["homogeneous"] else "fp64_general"
     achieved = main_res["achieved_GBs"]
+    tr = _traffic(workload, ttag)
+    traffic = None
+    if tr:
+        traffic = tr.get("dram_bytes_per_launch", tr.get("bytes_per_launch"))
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": _traffic(workload, ttag),
+        "traffic": traffic,
         "kernel": kname,
         "algorithmic_bytes_per_cell": main_res["bytes_per_cell"],
         "launch_ms": round(main_res["launch_ms"], 4),
+        "copy_ceiling_GBs": round(copy_gbs, 1),
+        "frac_of_copy": round(achieved / copy_gbs, 4),
+        "traffic_source": ("profiles/traffic.json " + ttag + ": TCC_EA0_RDREQ_DRAM + TCC_EA0_WRREQ_DRAM per dispatch, "
+                           "scaled by the copy kernel; FETCH_SIZE + WRITE_SIZE there too") if tr else None,
+        "traffic_fetch_write": tr.get("bytes_per_launch") if tr else None,
     }
 
     cpu = None
@@ -259,7 +336,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": workload, "n": list(case.n), "cells": cells, "cfl": 0.5,
                    "dt": main_res["dt"], "boundaries": "open x,y; reflecting z (reference semantics)",
-                   "path": "homogeneous" if main_res["homogeneous"] else "general"},
+                   "path": "homogeneous" if main_res["homogeneous"] else "general",
+                   "clock_settle_steps": main_res["settle_steps"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -59,10 +59,15 @@ subroutine set_initial_conditions
   use main_module
   use config_case_state
   implicit none
+  real*8, allocatable :: buf(:, :, :, :, :, :)
   read(case_unit) Nsqr
   read(case_unit) U
   read(case_unit) V
-  read(case_unit) E(1:nx, 1:ny, 1:nz, 1:nk, 1:nl, 1:nm)
+  ! a contiguous buffer: amdflang moves a strided section element by element
+  allocate(buf(nx, ny, nz, nk, nl, nm))
+  read(case_unit) buf
+  E(1:nx, 1:ny, 1:nz, 1:nk, 1:nl, 1:nm) = buf
+  deallocate(buf)
   close(case_unit)
 end subroutine set_initial_conditions
 

@@ -26,6 +26,7 @@ program main
   integer(8) :: c0, c1, crate, lim
   character(len=1024) :: arg, outfile
   real*8 :: secs
+  real*8, allocatable :: buf(:, :, :, :, :, :)
 
   outfile = '-'
   use_restart = .false.
@@ -78,9 +79,13 @@ program main
   if (use_restart) call write_restart
   call e_to_host
   if (trim(outfile) /= '-') then
+    ! through a contiguous buffer: amdflang writes a strided section element by element
+    allocate(buf(nx, ny, nz, nk, nl, nm))
+    buf = E(1:nx, 1:ny, 1:nz, 1:nk, 1:nl, 1:nm)
     open(newunit=iu, file=trim(outfile), access='stream', form='unformatted', status='replace')
-    write(iu) E(1:nx, 1:ny, 1:nz, 1:nk, 1:nl, 1:nm)
+    write(iu) buf
     close(iu)
+    deallocate(buf)
   endif
   call rt6d_check(rt6d_destroy(rt6d_handle), 'shutdown')
 end program main

@@ -436,18 +436,25 @@ def ensemble_benchmark(members=96, steps=300, precision=rt6d.FP64, seed=4242, al
         wall = time.perf_counter() - t1
     for m in ms:
         m.close()
+    # the same members one at a time, on the same clock as the batched run
+    # (host wall time from the first launch to the end of the last step)
     sample = reqs if alone_members is None else reqs[:alone_members]
-    t_alone, w_alone = 0.0, 0
+    t_alone, d_alone, w_alone = 0.0, 0.0, 0
     for r in sample:
         m = build_member(r)
         m.integrate(2)
         m.synchronize()
-        t_alone += m.integrate_timed(steps) * 1e-3
+        t0 = time.perf_counter()
+        d_alone += m.integrate_timed(steps)
+        m.synchronize()
+        t_alone += time.perf_counter() - t0
         w_alone += r.cells * steps
         m.close()
     out = {"members": members, "steps": steps, "cells": cells, "setup_s": setup, "schedule": [bpc, order], "tiles": tiles,
            "wall_s": wall, "device_ms": dev_ms, "value": cells * steps / wall,
-           "one_at_a_time": w_alone / t_alone if t_alone else None, "one_at_a_time_members": len(sample)}
+           "one_at_a_time": w_alone / t_alone if t_alone else None, "one_at_a_time_members": len(sample),
+           "one_at_a_time_wall_s": t_alone, "one_at_a_time_device_ms": d_alone,
+           "clock": "host wall time around the timed steps, batched and one at a time alike"}
     out["speedup_vs_one_at_a_time"] = out["value"] / out["one_at_a_time"] if out["one_at_a_time"] else None
     if log:
         log(f"ensemble {members} members x {steps} steps ({'fp32' if precision else 'fp64'}): "

@@ -57,14 +57,18 @@ subroutine set_initial_conditions
   use config_module
   implicit none
   integer :: iu
+  real*8, allocatable :: buf(:, :, :, :, :, :)
   open(newunit=iu, file='background.bin', access='stream', form='unformatted', status='old', action='read')
   read(iu) Nsqr
   read(iu) U
   read(iu) V
   close(iu)
+  allocate(buf(nx, ny, nz, nk, nl, nm))
   open(newunit=iu, file='E0.bin', access='stream', form='unformatted', status='old', action='read')
-  read(iu) E(1:nx, 1:ny, 1:nz, 1:nk, 1:nl, 1:nm)
+  read(iu) buf
   close(iu)
+  E(1:nx, 1:ny, 1:nz, 1:nk, 1:nl, 1:nm) = buf
+  deallocate(buf)
   if (forcing_ /= 0) then
     allocate(flux_bot(nx, ny, nk, nl, nm))
     open(newunit=iu, file='forcing.bin', access='stream', form='unformatted', status='old', action='read')
