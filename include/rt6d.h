@@ -97,7 +97,9 @@ extern "C" {
                                 homogeneous path, nx divisible by 4 and <= 64.
                                 An option whose kernel does not fit the grid runs the
                                 column kernel (two z cells per thread when nz is even), or
-                                the per-cell kernel where one field reaches 2 GiB */
+                                the per-cell kernel where one field reaches 2 GiB; the pair
+                                kernels (15-17, 21-23) address E one m plane at a time and
+                                fit until one m plane reaches 2 GiB */
 #define RT6D_OPT_SYNC 5      /* 1 (default): rt6d_step returns once its steps are done;
                                 0: rt6d_step returns once they are enqueued on the context's
                                 stream, and the next call that reads the device state
@@ -112,10 +114,11 @@ extern "C" {
 #define RT6D_Q_BYTES_PER_CELL 5 /* algorithmic HBM bytes per cell-update of the active step kernel */
 #define RT6D_Q_UNIFORM_XY 7  /* bit 0 (1): every x (y) spacing is the same, so the wave-local
                                 kernels take that axis's coefficients as scalars */
-#define RT6D_Q_KERNEL 6      /* the RT6D_OPT_KERNEL option rt6d_step runs: the set option, or
-                                what the auto choice resolves to (after calc_parameter, which
-                                decides the homogeneous path); an option that does not fit
-                                the grid runs the column kernel */
+#define RT6D_Q_KERNEL 6      /* the kernel rt6d_step runs, as an RT6D_OPT_KERNEL option: the
+                                set option or what the auto choice resolves to, after the
+                                fallback for a grid the option does not fit (the dispatch's
+                                own resolution; valid after calc_parameter, which decides
+                                the homogeneous path) */
 
 typedef struct rt6d_ctx rt6d_ctx;
 

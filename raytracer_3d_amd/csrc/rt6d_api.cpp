@@ -1239,9 +1239,9 @@ int rt6d_query(rt6d_ctx *c, int what, long long *value)
     case RT6D_Q_STEPS: *value = c->steps; return 0;
     case RT6D_Q_UNIFORM_XY: *value = c->g.uxy & 3; return 0;
     case RT6D_Q_KERNEL:
-        *value = c->opt_kernel ? c->opt_kernel
-                               : (c->prec == RT6D_FP64 ? auto_kernel<double>(c->g, c->homog, c->ncu)
-                                                       : auto_kernel<float>(c->g, c->homog, c->ncu));
+        // the kernel a step runs: the dispatch's own resolution of the option
+        *value = c->prec == RT6D_FP64 ? resolve_kernel<double>(c->g, c->homog, c->opt_kernel, c->ncu)
+                                      : resolve_kernel<float>(c->g, c->homog, c->opt_kernel, c->ncu);
         return 0;
     case RT6D_Q_BYTES_PER_CELL: {
         // E read + E write, plus the streamed coefficient fields on the general path

@@ -87,7 +87,7 @@ constexpr int kWLSlots = 128;
 // UXY: the x and y spacings are uniform (every face and cell of both axes
 // has the same dx), so that their coefficients are wave-uniform scalars
 // rather than per-lane values (fast arithmetic only).
-template <typename T, bool EXACT, int ZC, bool FULL, bool WL = false, bool UXY = false>
+template <typename T, bool EXACT, int ZC, bool FULL, bool WL = false, bool UXY = false, bool MP = false>
 __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, const ColX2Cfg &cc, uint32_t b)
 {
     static_assert(!(UXY && EXACT), "the exact arithmetic divides by the per-lane spacings");
@@ -124,6 +124,21 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
     const uint32_t planeB = (uint32_t)nxy * S;
     const uint32_t mB = (uint32_t)g.inner[5] * S;
     const uint32_t base0 = P0 * planeB;
+    // MP: E addressed one m plane at a time, through a buffer resource at the
+    // start of plane mm and 32-bit byte offsets inside it, for arrays beyond
+    // the reach of 32-bit offsets; else one resource per array
+    auto rin = [&](int mm) {
+        if constexpr (MP) return mk_rsrc(p.Ein + (size_t)(uint32_t)(mm - 1) * (uint32_t)g.inner[5], mB);
+        else return rE;
+    };
+    auto rout = [&](int mm) {
+        if constexpr (MP) return mk_rsrc(p.Eout + (size_t)(uint32_t)(mm - 1) * (uint32_t)g.inner[5], mB);
+        else return rO;
+    };
+    auto moff = [&](int mm) -> uint32_t {
+        if constexpr (MP) return 0u;
+        else return (uint32_t)(mm - 1) * mB;
+    };
     const bool ax = FULL || (g.act & 1), ay = FULL || (g.act & 2), az = FULL || (g.act & 4);
     const bool am = FULL || (g.act & 32);
     const double dtv = p.dt;
@@ -197,14 +212,14 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
         for (int q = 0; q < 4; ++q) sy[q] = (ys[q] - 1) * nxh + (x0 >> 1) - pw + H;
     }
     const int sown = xyc - pw + H;
-    auto poff = [&](int j, int mm) -> uint32_t { return base0 + (uint32_t)j * planeB + (uint32_t)(mm - 1) * mB; };
+    auto poff = [&](int j, int mm) -> uint32_t { return base0 + (uint32_t)j * planeB + moff(mm); };
     auto ldm = [&](int j, int mm) -> Pair<T> {
         mm = mm < 1 ? 1 : (mm > nm ? nm : mm);
-        return bld2<T>(rE, oc, poff(j, mm));
+        return bld2<T>(rin(mm), oc, poff(j, mm));
     };
     auto ldz = [&](int q, int mm) -> Pair<T> {
         const int mu = zmir[q] ? nm + 1 - mm : mm;
-        return bld2<T>(rE, oc, base0 + (uint32_t)(zsh[q] * (int)planeB) + (uint32_t)(mu - 1) * mB);
+        return bld2<T>(rin(mu), oc, base0 + (uint32_t)(zsh[q] * (int)planeB) + moff(mu));
     };
     constexpr int LA = ZC >= 4 ? 3 : 2;   // own-plane look-ahead, as in the column kernel
     Pair<T> wm1[ZC], w0[ZC], wp1[ZC], wp2[ZC], FLm[ZC];
@@ -219,14 +234,23 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
     Pair<T> zq[4];
     auto ldhal = [&](int j, int mm) -> Pair<T> {
         Pair<T> h{T(0), T(0)};
-        if (hval) h = bld2<T>(rE, ohal, (P0 + (uint32_t)j + (uint32_t)(mm - 1) * nzkl) * planeB);
+        if (hval) h = bld2<T>(rin(mm), ohal, poff(j, mm));
         return h;
     };
     if (am) {
 #pragma unroll
         for (int j = 0; j < ZC; ++j) {
             const T vm0 = sld(p.hom.vm, P0 + j);
-            FLm[j] = Pair<T>{vm0 * w0[j].a, vm0 * w0[j].b};   // m face 0: v*E(1)
+            // m face 0: v*E(1), the reference's formula over the zero-gradient
+            // ghosts in exact arithmetic (v*(E+E)*0.5 differs from v*E where
+            // the product is subnormal)
+            if constexpr (EXACT) {
+                const double d0 = sld(p.ax.dx[5], 0);
+                FLm[j] = Pair<T>{sb_flux_exact(vm0, p.dt, d0, w0[j].a, w0[j].a, w0[j].a, wp1[j].a),
+                                 sb_flux_exact(vm0, p.dt, d0, w0[j].b, w0[j].b, w0[j].b, wp1[j].b)};
+            } else {
+                FLm[j] = Pair<T>{vm0 * w0[j].a, vm0 * w0[j].b};
+            }
         }
     }
 
@@ -287,12 +311,12 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
         }
 #pragma unroll
         for (int j = 0; j < ZC; ++j) {
-            const uint32_t Pm = P0 + (uint32_t)j + (uint32_t)(m - 1) * nzkl;
             // raised wave priority while this cell pair's neighbour loads issue,
             // so that they leave early and other waves' arithmetic covers them
             // (binary64 at 24^3 x 16^3: 0.396 instead of 0.417 ms)
             __builtin_amdgcn_s_setprio(1);
-            const uint32_t pb = Pm * planeB;
+            const uint32_t pb = poff(j, m);   // cell pair (x, y, z0+j, k, l, m)
+            const __amdgpu_buffer_rsrc_t rEm = rin(m);
             const Pair<T> e = w0[j];
             T dEa = T(0), dEb = T(0);
             [[maybe_unused]] V2 *sl = nullptr;
@@ -322,8 +346,8 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
                     L = slot(sown - 1);
                     R = slot(sown + 1);
                 } else {
-                    L = bld2<T>(rE, oxl, pb);
-                    R = bld2<T>(rE, oxr, pb);
+                    L = bld2<T>(rEm, oxl, pb);
+                    R = bld2<T>(rEm, oxr, pb);
                 }
                 if (xfirst) L = Pair<T>{e.a, e.a};
                 if (xlast) R = Pair<T>{e.b, e.b};
@@ -365,10 +389,10 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
                     a3 = slot(sy[2]);
                     a4 = slot(sy[3]);
                 } else {
-                    a0 = bld2<T>(rE, oy[0], pb);
-                    a1 = bld2<T>(rE, oy[1], pb);
-                    a3 = bld2<T>(rE, oy[2], pb);
-                    a4 = bld2<T>(rE, oy[3], pb);
+                    a0 = bld2<T>(rEm, oy[0], pb);
+                    a1 = bld2<T>(rEm, oy[1], pb);
+                    a3 = bld2<T>(rEm, oy[2], pb);
+                    a4 = bld2<T>(rEm, oy[3], pb);
                 }
                 __builtin_amdgcn_s_setprio(0);
                 const T v0 = rv(j, kRecVy0), vb = rv(j, kRecVyn);
@@ -435,7 +459,7 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
                 en.a = e.a + (T)p.dt * dEa;
                 en.b = e.b + (T)p.dt * dEb;
             }
-            if (!WL || live) bst2<T>(en, rO, oc, pb);
+            if (!WL || live) bst2<T>(en, rout(m), oc, pb);
         }
 #pragma unroll
         for (int j = 0; j < ZC; ++j) {
@@ -447,10 +471,10 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
     }
 }
 
-template <typename T, bool EXACT, int ZC, bool FULL, bool WL = false, bool UXY = false>
+template <typename T, bool EXACT, int ZC, bool FULL, bool WL = false, bool UXY = false, bool MP = false>
 __global__ void __launch_bounds__(256) k_step_colx2(const Geo g, const StepPtrs<T> p, const ColX2Cfg cc, uint32_t nbl)
 {
     const uint32_t b = xcd_logical_block(blockIdx.x, nbl);
     if (b >= nbl) return;
-    colx2_body<T, EXACT, ZC, FULL, WL, UXY>(g, p, cc, b);
+    colx2_body<T, EXACT, ZC, FULL, WL, UXY, MP>(g, p, cc, b);
 }

@@ -575,6 +575,21 @@ hipError_t launch_colp(const Geo &g, const StepPtrs<T> &p, bool homog, hipStream
     return hipGetLastError();
 }
 
+// Byte-offset reach of the kernels that use buffer resources.  The pair
+// kernels address E one m plane at a time (a resource at the plane's start),
+// so only a plane has to stay below 2^31 bytes; the other column, plane and
+// march kernels address each array as a whole.
+template <typename T>
+inline bool whole_fits32(const Geo &g)
+{
+    return (uint64_t)g.ncells * 2u * sizeof(T) < (1ull << 32);
+}
+template <typename T>
+inline bool mplane_fits32(const Geo &g)
+{
+    return (uint64_t)(uint32_t)g.inner[5] * 2u * sizeof(T) < (1ull << 32);
+}
+
 // Pair kernel (two x cells per thread): homogeneous path, nx even.
 inline bool colx2_fits(const Geo &g, bool homog, int zc)
 {
@@ -611,6 +626,16 @@ hipError_t launch_colx2(const Geo &g, const StepPtrs<T> &p, hipStream_t stream)
     const uint64_t nthreads = (uint64_t)cc.nxyp * cc.nzb * g.n[3] * g.n[4];
     const uint32_t nbl = (uint32_t)((nthreads + 255) / 256);
     const uint32_t nbh = ((nbl + 7u) / 8u) * 8u;
+    // E beyond the reach of 32-bit offsets: addressed one m plane at a time
+    if (!whole_fits32<T>(g)) {
+        if (is_full<true>(g))
+            hipLaunchKernelGGL((k_step_colx2<T, EXACT, ZC, true, WL, false, true>), dim3(nbh), dim3(256), 0, stream, g,
+                               p, cc, nbl);
+        else
+            hipLaunchKernelGGL((k_step_colx2<T, EXACT, ZC, false, WL, false, true>), dim3(nbh), dim3(256), 0, stream, g,
+                               p, cc, nbl);
+        return hipGetLastError();
+    }
     // the wave-local kernels in fast arithmetic take uniform x and y spacings as scalars
     if constexpr (WL && !EXACT) {
         if ((g.uxy & 3) == 3 && is_full<true>(g)) {
@@ -677,20 +702,25 @@ inline int col_zc(const Geo &g, int kernel)
     return g.n[2] % 2 == 0 ? 2 : 1;
 }
 
-// One step.  kernel option: 0 auto (column kernel), 1 per-cell kernel,
-// 2/3 LDS marching kernel with 1/2 z cells per thread (nx*ny <= 1024),
-// 4/5/6 column kernel with 1/2/4 z cells per thread.
+// One step.  kernel option: 0 auto, 1 per-cell kernel, 2/3 LDS marching
+// kernel with 1/2 z cells per thread (nx*ny <= 1024), 4/5/6 column kernel
+// with 1/2/4 z cells per thread, 7/8 plane kernel, 9/10/11 column kernel
+// with LDS-staged y neighbours, 12-14 column kernel with paired y
+// accesses, 15-17 pair kernel, 18-20 binary32 quad kernel, 21-23 pair
+// kernel with the wave-local exchange, 24-26 quad kernel with it (1/2/4 z
+// cells per thread in each family).
 // The kernel option that the auto choice (option 0) resolves to.
 template <typename T>
 inline int auto_kernel(const Geo &g, bool homog, int ncu)
 {
-    // the column kernels address each array with 32-bit byte offsets
-    if ((uint64_t)g.ncells * 2u * sizeof(T) >= (1ull << 32)) return 1;
+    const bool whole = whole_fits32<T>(g);
     if (!homog) {
         // general path: the y neighbours from LDS (measured 15% faster at
         // 24^3 x 16^3 in binary64; the homogeneous path is faster without)
+        if (!whole) return 1;
         return g.n[2] % 2 == 0 ? 10 : 9;
     }
+    if (!mplane_fits32<T>(g)) return 1;
     // the pair kernel with the wave-local neighbour exchange where nx <= 32
     // (24^3 x 16^3: binary64 0.300 against the plain pair kernel's 0.327 ms,
     // binary32 0.142 against 0.195 for the quad kernel; 32^3 x 16^3: 0.652
@@ -710,81 +740,88 @@ inline int auto_kernel(const Geo &g, bool homog, int ncu)
     }
     // binary32 rows too wide for the exchange: quads, two z cells per thread
     if constexpr (sizeof(T) == 4) {
-        if (g.n[0] > 32 && colq_fits(g, true, 2) && colq_fills(g, 2, ncu)) return 19;
+        if (whole && g.n[0] > 32 && colq_fits(g, true, 2) && colq_fills(g, 2, ncu)) return 19;
     }
     // paired accesses, two z cells per thread where nz is even (binary64
     // 24^3 x 16^3: 10% faster than the column kernel)
     if (colx2_fits(g, true, 1)) return g.n[2] % 2 == 0 ? 16 : 15;
+    if (!whole) return 1;
     return g.n[2] % 2 == 0 ? 5 : 4;
 }
 
-template <typename T, bool EXACT>
-hipError_t launch_step_impl(const Geo &g, const StepPtrs<T> &p, bool homog, const LaunchCfg &lc0)
+// z cells per thread of option k's family (1, 2 or 4 by its position in the
+// family of three)
+inline int family_zc(int k, int first) { return k == first ? 1 : (k == first + 1 ? 2 : 4); }
+
+// The kernel a step with kernel option `opt` runs: the option itself where
+// the grid fits that kernel, else the kernel the dispatch falls back to.
+// The dispatch (launch_step_impl) and RT6D_Q_KERNEL both use it.
+template <typename T>
+inline int resolve_kernel(const Geo &g, bool homog, int opt, int ncu)
 {
-    LaunchCfg lc = lc0;
-    if (lc.kernel == 0) lc.kernel = auto_kernel<T>(g, homog, lc.ncu);
-    if (lc.kernel == 7 || lc.kernel == 8) {
-        const bool col_ok = (uint64_t)g.ncells * 2u * sizeof(T) < (1ull << 32);
-        if (col_ok && lc.kernel == 8 && plane_fits<T>(g, 2)) return launch_plane<T, EXACT, 2>(g, p, homog, lc.stream);
-        if (col_ok && plane_fits<T>(g, 1)) return launch_plane<T, EXACT, 1>(g, p, homog, lc.stream);
+    const bool whole = whole_fits32<T>(g), mplane = mplane_fits32<T>(g);
+    const int k = opt == 0 ? auto_kernel<T>(g, homog, ncu) : opt;
+    if ((k == 7 || k == 8) && whole) {
+        if (k == 8 && plane_fits<T>(g, 2)) return 8;
+        if (plane_fits<T>(g, 1)) return 7;
     }
-    if ((lc.kernel == 2 || lc.kernel == 3) && march_fits(g)) {
-        if (march_zc(g, lc.kernel) == 2) return launch_march<T, EXACT, 2>(g, p, homog, lc.stream);
-        return launch_march<T, EXACT, 1>(g, p, homog, lc.stream);
+    if ((k == 2 || k == 3) && whole && march_fits(g)) return march_zc(g, k) == 2 ? 3 : 2;
+    if (k >= 9 && k <= 11 && whole) {
+        const int zc = family_zc(k, 9);
+        if (zc == 4 && g.n[2] % 4 == 0) return 11;
+        if (zc >= 2 && g.n[2] % 2 == 0) return 10;
+        return 9;
     }
-    // the column kernel addresses each array with 32-bit byte offsets
-    const bool col_fits = (uint64_t)g.ncells * 2u * sizeof(T) < (1ull << 32);
-    if (lc.kernel >= 9 && lc.kernel <= 11 && col_fits) {
-        const int zc = lc.kernel == 9 ? 1 : (lc.kernel == 10 ? 2 : 4);
-        if (zc == 4 && g.n[2] % 4 == 0) return launch_col<T, EXACT, 4, 1>(g, p, homog, lc.stream);
-        if (zc >= 2 && g.n[2] % 2 == 0) return launch_col<T, EXACT, 2, 1>(g, p, homog, lc.stream);
-        return launch_col<T, EXACT, 1, 1>(g, p, homog, lc.stream);
+    if (k >= 15 && k <= 17 && mplane && colx2_fits(g, homog, family_zc(k, 15))) return k;
+    if (k >= 21 && k <= 23 && mplane && colx2wl_fits(g, homog, family_zc(k, 21))) return k;
+    if constexpr (sizeof(T) == 4) {
+        if (k >= 24 && k <= 26 && whole && colqwl_fits(g, homog, family_zc(k, 24))) return k;
+        if (k >= 18 && k <= 20 && whole && colq_fits(g, homog, family_zc(k, 18))) return k;
     }
-    if (lc.kernel >= 15 && lc.kernel <= 17 && col_fits) {
-        const int zc = lc.kernel == 15 ? 1 : (lc.kernel == 16 ? 2 : 4);
-        if (colx2_fits(g, homog, zc)) {
-            if (zc == 4) return launch_colx2<T, EXACT, 4>(g, p, lc.stream);
-            if (zc == 2) return launch_colx2<T, EXACT, 2>(g, p, lc.stream);
-            return launch_colx2<T, EXACT, 1>(g, p, lc.stream);
-        }
+    if (k >= 12 && k <= 14 && whole && colp_fits<T>(g, family_zc(k, 12))) return k;
+    if (k != 1 && whole) {
+        const int zc = col_zc(g, k);
+        return zc == 4 ? 6 : (zc == 2 ? 5 : 4);
     }
-    if (lc.kernel >= 21 && lc.kernel <= 23 && col_fits) {
-        const int zc = lc.kernel == 21 ? 1 : (lc.kernel == 22 ? 2 : 4);
-        if (colx2wl_fits(g, homog, zc)) {
-            if (zc == 4) return launch_colx2<T, EXACT, 4, true>(g, p, lc.stream);
-            if (zc == 2) return launch_colx2<T, EXACT, 2, true>(g, p, lc.stream);
-            return launch_colx2<T, EXACT, 1, true>(g, p, lc.stream);
-        }
+    return 1;
+}
+
+template <typename T, bool EXACT>
+hipError_t launch_step_impl(const Geo &g, const StepPtrs<T> &p, bool homog, const LaunchCfg &lc)
+{
+    const int k = resolve_kernel<T>(g, homog, lc.kernel, lc.ncu);
+    switch (k) {
+    case 2: return launch_march<T, EXACT, 1>(g, p, homog, lc.stream);
+    case 3: return launch_march<T, EXACT, 2>(g, p, homog, lc.stream);
+    case 4: return launch_col<T, EXACT, 1>(g, p, homog, lc.stream);
+    case 5: return launch_col<T, EXACT, 2>(g, p, homog, lc.stream);
+    case 6: return launch_col<T, EXACT, 4>(g, p, homog, lc.stream);
+    case 7: return launch_plane<T, EXACT, 1>(g, p, homog, lc.stream);
+    case 8: return launch_plane<T, EXACT, 2>(g, p, homog, lc.stream);
+    case 9: return launch_col<T, EXACT, 1, 1>(g, p, homog, lc.stream);
+    case 10: return launch_col<T, EXACT, 2, 1>(g, p, homog, lc.stream);
+    case 11: return launch_col<T, EXACT, 4, 1>(g, p, homog, lc.stream);
+    case 12: return launch_colp<T, EXACT, 1>(g, p, homog, lc.stream);
+    case 13: return launch_colp<T, EXACT, 2>(g, p, homog, lc.stream);
+    case 14: return launch_colp<T, EXACT, 4>(g, p, homog, lc.stream);
+    case 15: return launch_colx2<T, EXACT, 1>(g, p, lc.stream);
+    case 16: return launch_colx2<T, EXACT, 2>(g, p, lc.stream);
+    case 17: return launch_colx2<T, EXACT, 4>(g, p, lc.stream);
+    case 21: return launch_colx2<T, EXACT, 1, true>(g, p, lc.stream);
+    case 22: return launch_colx2<T, EXACT, 2, true>(g, p, lc.stream);
+    case 23: return launch_colx2<T, EXACT, 4, true>(g, p, lc.stream);
+    default: break;
     }
     if constexpr (sizeof(T) == 4) {
-        if (lc.kernel >= 24 && lc.kernel <= 26 && col_fits) {
-            const int zc = lc.kernel == 24 ? 1 : (lc.kernel == 25 ? 2 : 4);
-            if (colqwl_fits(g, homog, zc)) {
-                if (zc == 4) return launch_colq<EXACT, 4, true>(g, p, lc.stream);
-                if (zc == 2) return launch_colq<EXACT, 2, true>(g, p, lc.stream);
-                return launch_colq<EXACT, 1, true>(g, p, lc.stream);
-            }
+        switch (k) {
+        case 18: return launch_colq<EXACT, 1>(g, p, lc.stream);
+        case 19: return launch_colq<EXACT, 2>(g, p, lc.stream);
+        case 20: return launch_colq<EXACT, 4>(g, p, lc.stream);
+        case 24: return launch_colq<EXACT, 1, true>(g, p, lc.stream);
+        case 25: return launch_colq<EXACT, 2, true>(g, p, lc.stream);
+        case 26: return launch_colq<EXACT, 4, true>(g, p, lc.stream);
+        default: break;
         }
-        if (lc.kernel >= 18 && lc.kernel <= 20 && col_fits) {
-            const int zc = lc.kernel == 18 ? 1 : (lc.kernel == 19 ? 2 : 4);
-            if (colq_fits(g, homog, zc)) {
-                if (zc == 4) return launch_colq<EXACT, 4>(g, p, lc.stream);
-                if (zc == 2) return launch_colq<EXACT, 2>(g, p, lc.stream);
-                return launch_colq<EXACT, 1>(g, p, lc.stream);
-            }
-        }
-    }
-    if (lc.kernel >= 12 && lc.kernel <= 14 && col_fits && colp_fits<T>(g, lc.kernel == 12 ? 1 : (lc.kernel == 13 ? 2 : 4))) {
-        const int zc = lc.kernel == 12 ? 1 : (lc.kernel == 13 ? 2 : 4);
-        if (zc == 4) return launch_colp<T, EXACT, 4>(g, p, homog, lc.stream);
-        if (zc == 2) return launch_colp<T, EXACT, 2>(g, p, homog, lc.stream);
-        return launch_colp<T, EXACT, 1>(g, p, homog, lc.stream);
-    }
-    if (lc.kernel != 1 && col_fits) {
-        const int zc = col_zc(g, lc.kernel);
-        if (zc == 4) return launch_col<T, EXACT, 4>(g, p, homog, lc.stream);
-        if (zc == 2) return launch_col<T, EXACT, 2>(g, p, homog, lc.stream);
-        return launch_col<T, EXACT, 1>(g, p, homog, lc.stream);
     }
     const uint32_t nbl = (uint32_t)((g.ncells + lc.block - 1) / lc.block);
     const uint32_t nbh = ((nbl + 7u) / 8u) * 8u;

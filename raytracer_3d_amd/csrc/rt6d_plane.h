@@ -143,7 +143,11 @@ __global__ void __launch_bounds__(1024) k_step_plane(const Geo g, const StepPtrs
 #pragma unroll
         for (int j = 0; j < ZC; ++j) {
             const T vm0 = HOMOG ? sld(p.hom.vm, P0 + j) : bld<T>(rV[5], oc, poff(j, 1));
-            FLm[j] = vm0 * w0[j];   // m face 0: v*E(1)
+            // m face 0: v*E(1), the reference's formula over the zero-gradient
+            // ghosts in exact arithmetic (v*(E+E)*0.5 differs from v*E where
+            // the product is subnormal)
+            if constexpr (EXACT) FLm[j] = sb_flux_exact(vm0, p.dt, sld(p.ax.dx[5], 0), w0[j], w0[j], w0[j], wp1[j]);
+            else FLm[j] = vm0 * w0[j];
         }
     }
 

@@ -266,6 +266,29 @@ def test_auto_kernel_choice(n, precision, path, expected, gpu_available):
         assert m.kernel == 4
 
 
+@pytest.mark.parametrize("n, path, option, expected", [
+    ((24, 24, 23, 8, 8, 8), 0, 22, 4),     # two z cells per thread need nz even: column kernel, one
+    ((24, 24, 24, 8, 8, 8), 1, 16, 5),     # pairs need the homogeneous path: column kernel, two
+    ((24, 24, 24, 8, 8, 8), 1, 22, 5),
+    ((36, 36, 36, 8, 8, 8), 0, 22, 5),     # the wave-local exchange needs nx <= 32
+    ((36, 36, 36, 8, 8, 8), 0, 16, 16),    # plain pairs fit
+    ((24, 24, 24, 8, 8, 8), 0, 19, 5),     # quads are binary32 only
+], ids=["oddz-22", "general-16", "general-22", "wide-22", "wide-16", "fp64-19"])
+def test_kernel_query_names_the_fallback(n, path, option, expected, gpu_available):
+    """A forced option that does not fit the grid runs the dispatch's
+    fallback, and RT6D_Q_KERNEL names that kernel; the results are the
+    fallback's bits."""
+    case = cases.member_spec(3, 1, n=n)
+    outs = []
+    for k in (option, expected):
+        with rt6d.Model.from_case(case, path=path, kernel=k) as m:
+            m.calc_parameter()
+            assert m.kernel == expected, (k, m.kernel)
+            m.integrate(3, dt=m.cfl_dt(0.5))
+            outs.append(m.download("E"))
+    assert np.array_equal(outs[0], outs[1])
+
+
 # ----------------------------------------------------------- production size
 
 @pytest.fixture(scope="module")
@@ -354,24 +377,36 @@ def test_config0_packet_full_size(gpu_available):
         assert rel_l2(m.download("E"), ex) <= TOL32
 
 
-def test_beyond_32bit_offsets_uses_the_per_cell_kernel(gpu_available):
-    """48^3 x 16^3 (452,984,832 cells, 3.6 GB per binary64 field): the column
-    kernels address arrays with 32-bit byte offsets, so the binary64 step falls
-    back to the per-cell kernel while FP32 (1.8 GB fields) keeps the column
-    kernel; both stay finite, the budget closes, and they agree to FP32 accuracy."""
+def test_pair_kernel_beyond_2gib_fields(gpu_available):
+    """48^3 x 16^3 (452,984,832 cells, 3.6 GB per binary64 field): the pair
+    kernel addresses E one m plane at a time (226 MB), so the binary64 step
+    keeps it; it gives the per-cell kernel's bits in both arithmetic modes,
+    the budget closes, and FP32 (quad kernel, 1.8 GB fields) agrees to FP32
+    accuracy."""
     spec = cases.member_spec(11, 2, n=(48, 48, 48, 16, 16, 16))
-    with rt6d.Model.from_case(spec) as m64:
-        m64.calc_parameter()
-        assert m64.homogeneous and m64.query(rt6d.Q_CELLS) == 452984832
-        dt = m64.cfl_dt(0.5)
-        m64.integrate(4, dt=dt)
-        d = m64.diagnose(dt=dt)
-        e64 = m64.download("E")
+    out = {}
+    dt = None
+    for exact in (False, True):
+        for kernel in (0, 1):
+            with rt6d.Model.from_case(spec, kernel=kernel, exact=exact) as m64:
+                m64.calc_parameter()
+                assert m64.homogeneous and m64.query(rt6d.Q_CELLS) == 452984832
+                assert m64.kernel == (16 if kernel == 0 else 1), m64.kernel
+                dt = m64.cfl_dt(0.5) if dt is None else dt
+                m64.integrate(4, dt=dt)
+                if kernel == 0 and not exact:
+                    d = m64.diagnose(dt=dt)
+                out[exact, kernel] = m64.download("E")
+        assert np.array_equal(out[exact, 0], out[exact, 1]), f"exact={exact}"
+        del out[exact, 1]
+    e64 = out[False, 0]
+    assert rel_l2(out[True, 0], e64) <= TOL64
     terms = [d["eint"], d["mflint"]] + [d[k] for k in d if k.startswith("bfl_")]
     assert abs(d["residual"]) <= 1e-11 * sum(abs(t) for t in terms)
     assert np.all(np.isfinite(e64))
     with rt6d.Model.from_case(spec, precision=rt6d.FP32) as m32:
         m32.calc_parameter()
+        assert m32.kernel == 19
         m32.integrate(4, dt=dt)
         e32 = m32.download("E")
     assert rel_l2(e32, e64) <= TOL32
