@@ -76,14 +76,9 @@ extern "C" {
                                 even; the general path the column kernel with the y
                                 neighbours staged in LDS);
                                 1 per-cell kernel;
-                                2/3 LDS marching kernel, 1/2 z cells per thread (nx*ny <= 1024);
                                 4/5/6 column kernel, 1/2/4 z cells per thread;
-                                7/8 plane kernel with x/y face-flux exchange in LDS,
-                                1/2 z cells per thread (nx*ny <= 1024);
                                 9/10/11 column kernel with the y neighbours staged in
                                 LDS, 1/2/4 z cells per thread;
-                                12/13/14 the same with one block per (x,y) plane set, every
-                                y neighbour from LDS (nx*ny <= 1024);
                                 15/16/17 column kernel with two x cells per thread (paired
                                 accesses), 1/2/4 z cells per thread: homogeneous path with
                                 nx even;
@@ -92,9 +87,10 @@ extern "C" {
                                 homogeneous path with nx divisible by 4;
                                 21/22/23 the pair kernel with the x and y neighbour pairs
                                 exchanged through wave-local LDS slots: homogeneous path,
-                                nx even and <= 32;
-                                24/25/26 the quad kernel with the same exchange: binary32
-                                homogeneous path, nx divisible by 4 and <= 64.
+                                nx even and <= 32.
+                                Options 2, 3, 7, 8, 12-14 and 24-26 named kernels that were
+                                slower than these on every grid measured and were removed:
+                                setting one is RT6D_EINVAL.
                                 An option whose kernel does not fit the grid runs the
                                 column kernel (two z cells per thread when nz is even), or
                                 the per-cell kernel where one field reaches 2 GiB; the pair
@@ -267,7 +263,8 @@ int rt6d_synchronize(rt6d_ctx *ctx);
  * grid, background and dt -- with one launch per step and kernel variant
  * over a (member, block) work list (one block per task, or persistent blocks
  * taking tasks from work counters), two steps replayed from a captured HIP
- * graph.
+ * graph.  Members whose grids take the same kernel variant share one launch,
+ * so an ensemble of one variant steps with one launch per step.
  * Members keep their own state; while in an ensemble a context must not be
  * stepped on its own.  rt6d_ensemble_remove (or destroy) hands the context
  * back with its state current, so rt6d_download/rt6d_diagnose work on it.
@@ -311,6 +308,13 @@ int rt6d_ensemble_set_tiles(rt6d_ensemble *ens, int tiles);
 int rt6d_ensemble_tune(rt6d_ensemble *ens, int trial_steps, int *blocks_per_cu, int *order, int *tiles);
 /* number of members, total cells, steps taken by the ensemble */
 int rt6d_ensemble_info(rt6d_ensemble *ens, int *members, long long *cells, long long *steps);
+/* queries for rt6d_ensemble_query (after bringing the work list up to date
+ * with the current members and tiles) */
+#define RT6D_EQ_LAUNCHES 1   /* kernel launches per ensemble step: one per kernel variant
+                                among the members, nothing else (members whose grids take
+                                the same variant share one launch) */
+#define RT6D_EQ_TASKS 2      /* (member, block) tasks in the work list */
+int rt6d_ensemble_query(rt6d_ensemble *ens, int what, long long *value);
 
 #ifdef __cplusplus
 }

@@ -1195,26 +1195,12 @@ int rt6d_set_option(rt6d_ctx *c, int option, int value)
         c->opt_block = value;
         return 0;
     case RT6D_OPT_KERNEL:
-        if (value < 0 || value > 26) return fail(RT6D_EINVAL, "kernel option %d", value);
+        if (value < 0 || value > 23) return fail(RT6D_EINVAL, "kernel option %d", value);
+        if (kernel_option_removed(value))
+            return fail(RT6D_EINVAL, "kernel option %d names a removed kernel (slower than the column family)", value);
         if ((value == 16 && c->n[2] % 2 != 0) || (value == 17 && c->n[2] % 4 != 0))
             return fail(RT6D_EINVAL, "pair kernel: nz (%d) not divisible by its z cells per thread", c->n[2]);
-        if (value >= 12 && value <= 14) {
-            const int zc = value == 12 ? 1 : (value == 13 ? 2 : 4);
-            const bool fits = c->prec == RT6D_FP64 ? colp_fits<double>(c->g, zc) : colp_fits<float>(c->g, zc);
-            if (!fits)
-                return fail(RT6D_EINVAL, "plane-block column kernel needs nx*ny <= 1024, nz divisible by %d and "
-                            "its LDS buffers within 64 KiB (have nx*ny = %d)", zc, c->n[0] * c->n[1]);
-        }
-        if (value == 7 || value == 8) {
-            const int zc = value == 8 ? 2 : 1;
-            const bool fits = c->prec == RT6D_FP64 ? plane_fits<double>(c->g, zc) : plane_fits<float>(c->g, zc);
-            if (!fits)
-                return fail(RT6D_EINVAL, "plane kernel needs nx*ny <= 1024, nz divisible by %d and its LDS "
-                            "buffers within 64 KiB (have nx*ny = %d)", zc, c->n[0] * c->n[1]);
-        }
-        if ((value == 2 || value == 3) && !march_fits(c->g))
-            return fail(RT6D_EINVAL, "marching kernel needs nx*ny <= 1024 (have %d)", c->n[0] * c->n[1]);
-        if ((value == 3 || value == 5 || value == 10) && c->n[2] % 2 != 0)
+        if ((value == 5 || value == 10) && c->n[2] % 2 != 0)
             return fail(RT6D_EINVAL, "two z cells per thread need an even nz (have %d)", c->n[2]);
         if ((value == 6 || value == 11) && c->n[2] % 4 != 0)
             return fail(RT6D_EINVAL, "four z cells per thread need nz divisible by 4 (have %d)", c->n[2]);
@@ -1389,6 +1375,8 @@ int ens_rebuild_t(rt6d_ensemble *e)
     }
     if (mb) HIPCHK(hipMemcpyAsync(e->d_mem, mem.data(), mb, hipMemcpyHostToDevice, e->stream));
     if (tb) HIPCHK(hipMemcpyAsync(e->d_tasks, tasks.data(), tb, hipMemcpyHostToDevice, e->stream));
+    // both parity sets of work counters start at zero (outside any captured graph)
+    HIPCHK(hipMemsetAsync(e->d_ctr, 0, kEnsCounters * sizeof(uint32_t), e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     e->ntasks = (uint32_t)tasks.size();
     ens_drop_graphs(e);
@@ -1675,6 +1663,28 @@ int rt6d_ensemble_tune(rt6d_ensemble *e, int trial_steps, int *bpc_out, int *ord
     if (order_out) *order_out = bo;
     if (tiles_out) *tiles_out = bt;
     return 0;
+}
+
+int rt6d_ensemble_query(rt6d_ensemble *e, int what, long long *value)
+{
+    if (!e || !value) return fail(RT6D_EINVAL, "rt6d_ensemble_query: null argument");
+    int r;
+    if (e->dirty) {
+        HIPCHK(hipSetDevice(e->device));
+        if ((r = ens_settle(e))) return r;
+        r = e->prec == RT6D_FP64 ? ens_rebuild_t<double>(e) : ens_rebuild_t<float>(e);
+        if (r) return r;
+    }
+    switch (what) {
+    case RT6D_EQ_LAUNCHES: {
+        long long n = 0;
+        for (int v = 0; v < kEnsVariants; ++v) n += e->cnt[v] ? 1 : 0;
+        *value = n;
+        return 0;
+    }
+    case RT6D_EQ_TASKS: *value = (long long)e->ntasks; return 0;
+    default: return fail(RT6D_EINVAL, "rt6d_ensemble_query: unknown query %d", what);
+    }
 }
 
 int rt6d_ensemble_info(rt6d_ensemble *e, int *members, long long *cells, long long *steps)

@@ -45,10 +45,7 @@ __device__ __forceinline__ void bst4(const Quad &v, __amdgpu_buffer_rsrc_t r, ui
 // nx/4).
 using ColQCfg = ColX2Cfg;
 
-// WL: the x and y neighbour quads through wave-local LDS slots, as in the
-// pair kernel (kWLSlots slots per wavefront and z cell: H = nx/2 quads, two
-// rows, on each side of the wave's 64 quads; nx <= 64).
-template <bool EXACT, int ZC, bool FULL, bool WL = false>
+template <bool EXACT, int ZC, bool FULL>
 __device__ __forceinline__ void colq_body(const Geo &g, const StepPtrs<float> &p, const ColQCfg &cc, uint32_t b)
 {
     using T = float;
@@ -63,15 +60,9 @@ __device__ __forceinline__ void colq_body(const Geo &g, const StepPtrs<float> &p
     const int bk = (int)kl - bl * nk;
     if (bl >= nl) return;          // whole wavefronts beyond the grid
     const int nquads = nxq * ny;
-    if constexpr (!WL) {
-        if (xyq >= nquads) return;   // padding lanes (no barriers below)
-    }
-    // WL: padding lanes stay to load halo quads; they compute on the plane's
-    // last quad and store nothing
-    const bool live = xyq < nquads;
-    const int xyc = live ? xyq : nquads - 1;
-    const int y0 = (int)fdiv((uint32_t)xyc, cc.dxh);
-    const int x0 = 4 * (xyc - y0 * nxq);   // 0-based first cell of the quad
+    if (xyq >= nquads) return;   // padding lanes (no barriers below)
+    const int y0 = (int)fdiv((uint32_t)xyq, cc.dxh);
+    const int x0 = 4 * (xyq - y0 * nxq);   // 0-based first cell of the quad
     const int x = x0 + 1, y = y0 + 1;      // 1-based: the quad is (x .. x+3)
     const int z0 = zb * ZC + 1, k = bk + 1, l = bl + 1;
     constexpr uint32_t S = sizeof(T);
@@ -133,28 +124,6 @@ __device__ __forceinline__ void colq_body(const Geo &g, const StepPtrs<float> &p
             zmir[q] = mir;
         }
     }
-    using V4 = float __attribute__((ext_vector_type(4)));
-    using V2 = float __attribute__((ext_vector_type(2)));
-    const int H = 2 * nxq;                         // quads in two rows
-    const int lane = (int)(threadIdx.x & 63u), wv = (int)(threadIdx.x >> 6);
-    const int pw = xyq & ~63;
-    int hslot = 0, hp = -1;
-    if (lane < H) { hp = pw - H + lane; hslot = lane; }
-    else if (lane < 2 * H) { hp = pw + 64 + (lane - H); hslot = 64 + lane; }
-    const bool hval = WL && hp >= 0 && hp < nquads;
-    uint32_t ohal = 0;
-    if (hval) {
-        const int yh = hp / nxq;
-        ohal = (uint32_t)(4 * (hp - yh * nxq) + nx * yh) * S;
-    }
-    int sy[4] = {0, 0, 0, 0};
-    if constexpr (WL) {
-        const int ys[4] = {y - 2 < 1 ? 1 : y - 2, y - 1 < 1 ? 1 : y - 1, y + 1 > ny ? ny : y + 1,
-                           y + 2 > ny ? ny : y + 2};
-#pragma unroll
-        for (int q = 0; q < 4; ++q) sy[q] = (ys[q] - 1) * nxq + (x0 >> 2) - pw + H;
-    }
-    const int sown = xyc - pw + H;
     auto poff = [&](int j, int mm) -> uint32_t { return base0 + (uint32_t)j * planeB + (uint32_t)(mm - 1) * mB; };
     auto ldm = [&](int j, int mm) -> Quad {
         mm = mm < 1 ? 1 : (mm > nm ? nm : mm);
@@ -195,14 +164,6 @@ __device__ __forceinline__ void colq_body(const Geo &g, const StepPtrs<float> &p
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) zq[q] = az ? ldz(q, m) : Quad{{T(0), T(0), T(0), T(0)}};
-        Quad hq[ZC];
-        if constexpr (WL) {
-#pragma unroll
-            for (int j = 0; j < ZC; ++j) {
-                hq[j] = Quad{{T(0), T(0), T(0), T(0)}};
-                if (hval) hq[j] = bld4(rE, ohal, (P0 + (uint32_t)j + (uint32_t)(m - 1) * nzkl) * planeB);
-            }
-        }
 
         T Fz[4][ZC + 1];
         if (az) {
@@ -234,43 +195,8 @@ __device__ __forceinline__ void colq_body(const Geo &g, const StepPtrs<float> &p
             const uint32_t q2 = q20 + (uint32_t)j;
             const Quad e = w0[j];
             T dE[4] = {T(0), T(0), T(0), T(0)};
-            [[maybe_unused]] V4 *sl = nullptr;
-            if constexpr (WL) {
-                __shared__ V4 sW[4][ZC][kWLSlots];
-                sl = &sW[wv][j][0];
-                __builtin_amdgcn_wave_barrier();   // the previous step's reads stay above
-                V4 own;
-                own.x = e.c[0];
-                own.y = e.c[1];
-                own.z = e.c[2];
-                own.w = e.c[3];
-                sl[sown] = own;
-                if (hval) {
-                    V4 h;
-                    h.x = hq[j].c[0];
-                    h.y = hq[j].c[1];
-                    h.z = hq[j].c[2];
-                    h.w = hq[j].c[3];
-                    sl[hslot] = h;
-                }
-                __builtin_amdgcn_wave_barrier();   // the reads below stay below
-            }
-            auto slot = [&](int i) -> Quad {
-                const V4 v = sl[i];
-                return Quad{{v.x, v.y, v.z, v.w}};
-            };
             if (ax) {
-                Pair<T> L, R;
-                if constexpr (WL) {
-                    // the upper half of the left quad, the lower half of the right one
-                    const V2 lv = reinterpret_cast<const V2 *>(sl + (sown - 1))[1];
-                    const V2 rv = reinterpret_cast<const V2 *>(sl + (sown + 1))[0];
-                    L = Pair<T>{lv.x, lv.y};
-                    R = Pair<T>{rv.x, rv.y};
-                } else {
-                    L = bld2<T>(rE, oxl, pb);
-                    R = bld2<T>(rE, oxr, pb);
-                }
+                Pair<T> L = bld2<T>(rE, oxl, pb), R = bld2<T>(rE, oxr, pb);
                 if (xfirst) L = Pair<T>{e.c[0], e.c[0]};
                 if (xlast) R = Pair<T>{e.c[3], e.c[3]};
                 const T v0 = sld(p.hom.vx3, q2), vi = sld(p.hom.vx3, p.hom.n4 + q2), vb = sld(p.hom.vx3, 2 * p.hom.n4 + q2);
@@ -286,18 +212,8 @@ __device__ __forceinline__ void colq_body(const Geo &g, const StepPtrs<float> &p
                 for (int c = 0; c < 4; ++c) dE[c] = div_term<T, EXACT>(dE[c], F[c + 1], F[c], dxf[c + 1], rx[c]);
             }
             if (ay) {
-                Quad a0, a1, a3, a4;
-                if constexpr (WL) {
-                    a0 = slot(sy[0]);
-                    a1 = slot(sy[1]);
-                    a3 = slot(sy[2]);
-                    a4 = slot(sy[3]);
-                } else {
-                    a0 = bld4(rE, oy[0], pb);
-                    a1 = bld4(rE, oy[1], pb);
-                    a3 = bld4(rE, oy[2], pb);
-                    a4 = bld4(rE, oy[3], pb);
-                }
+                const Quad a0 = bld4(rE, oy[0], pb), a1 = bld4(rE, oy[1], pb);
+                const Quad a3 = bld4(rE, oy[2], pb), a4 = bld4(rE, oy[3], pb);
                 __builtin_amdgcn_s_setprio(0);
                 const T v0 = sld(p.hom.vy3, q2), vi = sld(p.hom.vy3, p.hom.n4 + q2), vb = sld(p.hom.vy3, 2 * p.hom.n4 + q2);
                 const T vL = y == 1 ? v0 : vi, vR = y == ny ? vb : vi;
@@ -332,7 +248,7 @@ __device__ __forceinline__ void colq_body(const Geo &g, const StepPtrs<float> &p
                 const T d = dE[c] - e.c[c] * lam;
                 en.c[c] = e.c[c] + (T)p.dt * d;
             }
-            if (!WL || live) bst4(en, rO, oc, pb);
+            bst4(en, rO, oc, pb);
         }
 #pragma unroll
         for (int j = 0; j < ZC; ++j) {
@@ -344,10 +260,10 @@ __device__ __forceinline__ void colq_body(const Geo &g, const StepPtrs<float> &p
     }
 }
 
-template <bool EXACT, int ZC, bool FULL, bool WL = false>
+template <bool EXACT, int ZC, bool FULL>
 __global__ void __launch_bounds__(256) k_step_colq(const Geo g, const StepPtrs<float> p, const ColQCfg cc, uint32_t nbl)
 {
     const uint32_t b = xcd_logical_block(blockIdx.x, nbl);
     if (b >= nbl) return;
-    colq_body<EXACT, ZC, FULL, WL>(g, p, cc, b);
+    colq_body<EXACT, ZC, FULL>(g, p, cc, b);
 }

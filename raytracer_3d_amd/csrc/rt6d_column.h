@@ -94,10 +94,8 @@ inline bool is_full(const Geo &g) { return (g.act & full_mask<HOMOG>()) == full_
 // YM selects where the y neighbours come from: 0 memory; 1 the block's copy
 // of its 256 (x,y) positions in LDS (one barrier per m step, two buffers
 // alternating with m) where the neighbour lies inside the block, else
-// memory; 2 the same with one block per (x,y) plane set (blockDim = the
-// padded plane size, dynamic LDS), so that every y neighbour is in LDS.
-// With YM != 0 lanes beyond the grid stay to the end (for the barriers) on
-// clamped in-bounds coordinates and store nothing.
+// memory.  With YM != 0 lanes beyond the grid stay to the end (for the
+// barriers) on clamped in-bounds coordinates and store nothing.
 template <typename T, bool EXACT, bool HOMOG, int ZC, bool FULL = false, int YM = 0>
 __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, const ColCfg &cc, uint32_t b)
 {
@@ -259,14 +257,10 @@ __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, con
         for (int q = 0; q < 4; ++q) znx[q] = (az && m < nm) ? ldz(q, m + 1) : T(0);
 
         T *sb = nullptr;
-        constexpr int NT1 = 256;
-        const int NT = YM == 2 ? (int)blockDim.x : NT1;
+        constexpr int NT = 256;
         if constexpr (YM == 1) {
-            __shared__ T sEy[2][ZC][NT1];
+            __shared__ T sEy[2][ZC][NT];
             sb = &sEy[m & 1][0][0];
-        } else if constexpr (YM == 2) {
-            extern __shared__ unsigned char col_smem[];
-            sb = reinterpret_cast<T *>(col_smem) + (size_t)(m & 1) * ZC * NT;
         }
         if constexpr (YM != 0) {
 #pragma unroll
@@ -323,7 +317,7 @@ __device__ __forceinline__ void col_body(const Geo &g, const StepPtrs<T> &p, con
                     a1 = sj[ly[1]];
                     a3 = sj[ly[2]];
                     a4 = sj[ly[3]];
-                    if (YM == 1 && !(yin[0] && yin[1] && yin[2] && yin[3])) {   // rows next to the block's ends
+                    if (!(yin[0] && yin[1] && yin[2] && yin[3])) {   // rows next to the block's ends
                         if (!yin[0]) a0 = bld<T>(rE, oy[0], pb);
                         if (!yin[1]) a1 = bld<T>(rE, oy[1], pb);
                         if (!yin[2]) a3 = bld<T>(rE, oy[2], pb);
@@ -411,12 +405,3 @@ __global__ void __launch_bounds__(256) k_step_col(const Geo g, const StepPtrs<T>
     col_body<T, EXACT, HOMOG, ZC, FULL, YM>(g, p, cc, b);
 }
 
-// One block per (x,y) plane set: blockDim = the padded plane size (<= 1024),
-// the y neighbours all from LDS (YM = 2).
-template <typename T, bool EXACT, bool HOMOG, int ZC, bool FULL>
-__global__ void __launch_bounds__(1024) k_step_colp(const Geo g, const StepPtrs<T> p, const ColCfg cc, uint32_t nbl)
-{
-    const uint32_t b = xcd_logical_block(blockIdx.x, nbl);
-    if (b >= nbl) return;   // uniform over the block
-    col_body<T, EXACT, HOMOG, ZC, FULL, 2>(g, p, cc, b);
-}

@@ -14,8 +14,13 @@
 //    neighbouring blocks of one member share that XCD's L2);
 //  * persistent: a grid of blocks_per_cu blocks per compute unit; the blocks
 //    of each XCD take the tasks of that XCD's eighth of the list from a work
-//    counter until it is exhausted (every block reaches the end).  The
-//    counters are cleared by a small kernel at the start of every step.
+//    counter until it is exhausted (every block reaches the end).  The work
+//    counters come in two sets, one per step parity: a step takes its tasks
+//    from its parity's set and its first block clears the other set, which
+//    the previous step used and the next step will use (steps of one
+//    ensemble run in order on its stream).  The host clears both sets when
+//    the work list is rebuilt, so a step is one launch per variant with no
+//    clearing kernel or memset between launches.
 #pragma once
 
 template <typename T>
@@ -39,12 +44,8 @@ struct EnsTask {
 };
 
 constexpr int kEnsVariants = 128;
-constexpr int kEnsCounters = kEnsVariants * 8;   // one work counter per variant and XCD
-
-static __global__ void k_ens_reset(uint32_t *ctr)
-{
-    if (threadIdx.x < kEnsCounters) ctr[threadIdx.x] = 0;
-}
+constexpr int kEnsCounterSet = kEnsVariants * 8;   // one work counter per variant and XCD
+constexpr int kEnsCounters = 2 * kEnsCounterSet;   // a set per step parity
 
 template <typename T, bool HOMOG, int ZC, bool FULL, bool X2, bool Q, bool W, bool UXY>
 __device__ __forceinline__ void ens_task(const EnsMember<T> &M, uint32_t parity, uint32_t block)
@@ -72,12 +73,15 @@ __global__ void __launch_bounds__(256) k_ens_step(const EnsMember<T> *__restrict
         ens_task<T, HOMOG, ZC, FULL, X2, Q, W, UXY>(mem[tk.member], parity, tk.block);
     } else {
         __shared__ uint32_t s_task;
+        // ctr: this variant's counters in the set of parity 0
+        uint32_t *const mine = ctr + (parity & 1u) * kEnsCounterSet;
+        if (blockIdx.x == 0 && threadIdx.x < 8) ctr[((parity & 1u) ^ 1u) * kEnsCounterSet + threadIdx.x] = 0u;
         const uint32_t xcd = blockIdx.x & 7u;
         const uint32_t per = (ntasks + 7u) / 8u;
         const uint32_t t0 = xcd * per;
         const uint32_t t1 = t0 + per < ntasks ? t0 + per : ntasks;
         for (;;) {
-            if (threadIdx.x == 0) s_task = t0 + atomicAdd(&ctr[xcd], 1u);
+            if (threadIdx.x == 0) s_task = t0 + atomicAdd(&mine[xcd], 1u);
             __syncthreads();
             const uint32_t t = s_task;
             __syncthreads();   // every thread has read s_task before thread 0 writes it again

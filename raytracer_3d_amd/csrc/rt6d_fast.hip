@@ -69,7 +69,6 @@ hipError_t launch_ensemble(const EnsMember<T> *mem, const EnsTask *tasks, const 
                            const uint32_t cnt[kEnsVariants], uint32_t parity, uint32_t persist_blocks, uint32_t *ctr,
                            hipStream_t s)
 {
-    if (persist_blocks) hipLaunchKernelGGL(k_ens_reset, dim3(1), dim3(kEnsCounters), 0, s, ctr);
     for (int v = 0; v < kEnsVariants; ++v) {
         if (!cnt[v]) continue;
         const EnsTask *tv = tasks + off[v];

@@ -183,8 +183,6 @@ __device__ __forceinline__ T div_term(T dE, T FR, T FL, double dxc, double rdxc)
 }
 
 #include "rt6d_column.h"
-#include "rt6d_march.h"
-#include "rt6d_plane.h"
 #include "rt6d_colx2.h"
 #include "rt6d_colq.h"
 #include "rt6d_ensemble.h"
@@ -444,73 +442,10 @@ __global__ void __launch_bounds__(256) k_diag_columns(const Geo g, const StepPtr
 
 struct LaunchCfg {
     int block;          // threads per block of the per-cell kernel
-    int kernel;         // 0 auto, 1 per-cell kernel, 2 marching kernel
+    int kernel;         // RT6D_OPT_KERNEL option (0 auto)
     int ncu;            // compute units of the device
     hipStream_t stream;
 };
-
-inline bool march_fits(const Geo &g) { return g.n[0] * g.n[1] <= 1024; }
-
-// z cells per thread of the marching kernel for a kernel option
-inline int march_zc(const Geo &g, int kernel)
-{
-    if (kernel == 2) return 1;
-    return (g.n[2] % 2 == 0) ? 2 : 1;
-}
-
-
-template <typename T, bool EXACT, int ZC>
-hipError_t launch_march(const Geo &g, const StepPtrs<T> &p, bool homog, hipStream_t stream)
-{
-    const uint32_t nxy = (uint32_t)g.n[0] * g.n[1];
-    const uint32_t nthr = ((nxy + 63u) / 64u) * 64u;
-    MarchCfg mc;
-    mc.nzb = g.n[2] / ZC;
-    mc.dzb = make_fastdiv((uint32_t)mc.nzb);
-    const uint32_t nbl = (uint32_t)mc.nzb * g.n[3] * g.n[4];
-    const uint32_t nbh = ((nbl + 7u) / 8u) * 8u;
-    const size_t smem = 2 * ZC * (size_t)nthr * sizeof(T);
-    if (homog) {
-        if (is_full<true>(g))
-            hipLaunchKernelGGL((k_step_march<T, EXACT, true, ZC, true>), dim3(nbh), dim3(nthr), smem, stream, g, p, mc, nbl);
-        else
-            hipLaunchKernelGGL((k_step_march<T, EXACT, true, ZC, false>), dim3(nbh), dim3(nthr), smem, stream, g, p, mc, nbl);
-    } else {
-        if (is_full<false>(g))
-            hipLaunchKernelGGL((k_step_march<T, EXACT, false, ZC, true>), dim3(nbh), dim3(nthr), smem, stream, g, p, mc, nbl);
-        else
-            hipLaunchKernelGGL((k_step_march<T, EXACT, false, ZC, false>), dim3(nbh), dim3(nthr), smem, stream, g, p, mc, nbl);
-    }
-    return hipGetLastError();
-}
-
-// The plane kernel keeps one thread per (x,y) column in one block and two
-// LDS buffers of x and y face fluxes for ZC planes (at most 64 KiB).
-template <typename T>
-inline bool plane_fits(const Geo &g, int zc)
-{
-    const uint32_t nxy = (uint32_t)g.n[0] * g.n[1];
-    const uint32_t nthr = ((nxy + 63u) / 64u) * 64u;
-    return nxy <= 1024 && g.n[2] % zc == 0 && 4ull * zc * nthr * sizeof(T) <= 65536ull;
-}
-
-template <typename T, bool EXACT, int ZC>
-hipError_t launch_plane(const Geo &g, const StepPtrs<T> &p, bool homog, hipStream_t stream)
-{
-    const uint32_t nxy = (uint32_t)g.n[0] * g.n[1];
-    const uint32_t nthr = ((nxy + 63u) / 64u) * 64u;
-    PlaneCfg pc;
-    pc.nzb = g.n[2] / ZC;
-    pc.dzb = make_fastdiv((uint32_t)pc.nzb);
-    const uint32_t nbl = (uint32_t)pc.nzb * g.n[3] * g.n[4];
-    const uint32_t nbh = ((nbl + 7u) / 8u) * 8u;
-    const size_t smem = 4 * ZC * (size_t)nthr * sizeof(T);
-    if (homog)
-        hipLaunchKernelGGL((k_step_plane<T, EXACT, true, ZC>), dim3(nbh), dim3(nthr), smem, stream, g, p, pc, nbl);
-    else
-        hipLaunchKernelGGL((k_step_plane<T, EXACT, false, ZC>), dim3(nbh), dim3(nthr), smem, stream, g, p, pc, nbl);
-    return hipGetLastError();
-}
 
 template <typename T, bool EXACT, int ZC, int YM = 0>
 hipError_t launch_col(const Geo &g, const StepPtrs<T> &p, bool homog, hipStream_t stream)
@@ -538,47 +473,10 @@ hipError_t launch_col(const Geo &g, const StepPtrs<T> &p, bool homog, hipStream_
     return hipGetLastError();
 }
 
-// One block per (x,y) plane set: the padded plane fits one block and the
-// two LDS buffers of ZC planes fit 64 KiB.
-template <typename T>
-inline bool colp_fits(const Geo &g, int zc)
-{
-    const uint32_t nxy = (uint32_t)g.n[0] * g.n[1];
-    const uint32_t nthr = ((nxy + 63u) / 64u) * 64u;
-    return nthr <= 1024 && g.n[2] % zc == 0 && 2ull * zc * nthr * sizeof(T) <= 65536ull;
-}
-
-template <typename T, bool EXACT, int ZC>
-hipError_t launch_colp(const Geo &g, const StepPtrs<T> &p, bool homog, hipStream_t stream)
-{
-    const uint32_t nxy = (uint32_t)g.n[0] * g.n[1];
-    ColCfg cc;
-    cc.nxyp = (int)(((nxy + 63u) / 64u) * 64u);
-    cc.nzb = g.n[2] / ZC;
-    cc.dxyp = make_fastdiv((uint32_t)cc.nxyp);
-    cc.dzb = make_fastdiv((uint32_t)cc.nzb);
-    const uint32_t nbl = (uint32_t)cc.nzb * g.n[3] * g.n[4];
-    const uint32_t nbh = ((nbl + 7u) / 8u) * 8u;
-    const size_t smem = 2 * ZC * (size_t)cc.nxyp * sizeof(T);
-    const dim3 blk((uint32_t)cc.nxyp);
-    if (homog) {
-        if (is_full<true>(g))
-            hipLaunchKernelGGL((k_step_colp<T, EXACT, true, ZC, true>), dim3(nbh), blk, smem, stream, g, p, cc, nbl);
-        else
-            hipLaunchKernelGGL((k_step_colp<T, EXACT, true, ZC, false>), dim3(nbh), blk, smem, stream, g, p, cc, nbl);
-    } else {
-        if (is_full<false>(g))
-            hipLaunchKernelGGL((k_step_colp<T, EXACT, false, ZC, true>), dim3(nbh), blk, smem, stream, g, p, cc, nbl);
-        else
-            hipLaunchKernelGGL((k_step_colp<T, EXACT, false, ZC, false>), dim3(nbh), blk, smem, stream, g, p, cc, nbl);
-    }
-    return hipGetLastError();
-}
-
 // Byte-offset reach of the kernels that use buffer resources.  The pair
 // kernels address E one m plane at a time (a resource at the plane's start),
-// so only a plane has to stay below 2^31 bytes; the other column, plane and
-// march kernels address each array as a whole.
+// so only a plane has to stay below 2^31 bytes; the other column kernels
+// address each array as a whole.
 template <typename T>
 inline bool whole_fits32(const Geo &g)
 {
@@ -670,10 +568,7 @@ inline bool colq_fills(const Geo &g, int zc, int ncu)
     return 2 * waves >= 5ull * 4u * (uint64_t)(ncu > 0 ? ncu : 256);
 }
 
-// The wave-local LDS variant of the quad kernel: nx <= 64.
-inline bool colqwl_fits(const Geo &g, bool homog, int zc) { return colq_fits(g, homog, zc) && g.n[0] <= 64; }
-
-template <bool EXACT, int ZC, bool WL = false>
+template <bool EXACT, int ZC>
 hipError_t launch_colq(const Geo &g, const StepPtrs<float> &p, hipStream_t stream)
 {
     const uint32_t nquads = (uint32_t)(g.n[0] / 4) * g.n[1];
@@ -687,9 +582,9 @@ hipError_t launch_colq(const Geo &g, const StepPtrs<float> &p, hipStream_t strea
     const uint32_t nbl = (uint32_t)((nthreads + 255) / 256);
     const uint32_t nbh = ((nbl + 7u) / 8u) * 8u;
     if (is_full<true>(g))
-        hipLaunchKernelGGL((k_step_colq<EXACT, ZC, true, WL>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
+        hipLaunchKernelGGL((k_step_colq<EXACT, ZC, true>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
     else
-        hipLaunchKernelGGL((k_step_colq<EXACT, ZC, false, WL>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
+        hipLaunchKernelGGL((k_step_colq<EXACT, ZC, false>), dim3(nbh), dim3(256), 0, stream, g, p, cc, nbl);
     return hipGetLastError();
 }
 
@@ -702,13 +597,12 @@ inline int col_zc(const Geo &g, int kernel)
     return g.n[2] % 2 == 0 ? 2 : 1;
 }
 
-// One step.  kernel option: 0 auto, 1 per-cell kernel, 2/3 LDS marching
-// kernel with 1/2 z cells per thread (nx*ny <= 1024), 4/5/6 column kernel
-// with 1/2/4 z cells per thread, 7/8 plane kernel, 9/10/11 column kernel
-// with LDS-staged y neighbours, 12-14 column kernel with paired y
-// accesses, 15-17 pair kernel, 18-20 binary32 quad kernel, 21-23 pair
-// kernel with the wave-local exchange, 24-26 quad kernel with it (1/2/4 z
-// cells per thread in each family).
+// One step.  kernel option: 0 auto, 1 per-cell kernel, 4/5/6 column kernel
+// with 1/2/4 z cells per thread, 9/10/11 column kernel with LDS-staged y
+// neighbours, 15-17 pair kernel, 18-20 binary32 quad kernel, 21-23 pair
+// kernel with the wave-local exchange (1/2/4 z cells per thread in each
+// family).  Options 2, 3, 7, 8, 12-14 and 24-26 named kernels that measured
+// slower than these on every grid and were removed; they are rejected.
 // The kernel option that the auto choice (option 0) resolves to.
 template <typename T>
 inline int auto_kernel(const Geo &g, bool homog, int ncu)
@@ -749,6 +643,9 @@ inline int auto_kernel(const Geo &g, bool homog, int ncu)
     return g.n[2] % 2 == 0 ? 5 : 4;
 }
 
+// Kernel options of kernels that were removed (slower on every grid measured).
+inline bool kernel_option_removed(int k) { return k == 2 || k == 3 || k == 7 || k == 8 || (k >= 12 && k <= 14) || k >= 24; }
+
 // z cells per thread of option k's family (1, 2 or 4 by its position in the
 // family of three)
 inline int family_zc(int k, int first) { return k == first ? 1 : (k == first + 1 ? 2 : 4); }
@@ -761,11 +658,6 @@ inline int resolve_kernel(const Geo &g, bool homog, int opt, int ncu)
 {
     const bool whole = whole_fits32<T>(g), mplane = mplane_fits32<T>(g);
     const int k = opt == 0 ? auto_kernel<T>(g, homog, ncu) : opt;
-    if ((k == 7 || k == 8) && whole) {
-        if (k == 8 && plane_fits<T>(g, 2)) return 8;
-        if (plane_fits<T>(g, 1)) return 7;
-    }
-    if ((k == 2 || k == 3) && whole && march_fits(g)) return march_zc(g, k) == 2 ? 3 : 2;
     if (k >= 9 && k <= 11 && whole) {
         const int zc = family_zc(k, 9);
         if (zc == 4 && g.n[2] % 4 == 0) return 11;
@@ -775,10 +667,8 @@ inline int resolve_kernel(const Geo &g, bool homog, int opt, int ncu)
     if (k >= 15 && k <= 17 && mplane && colx2_fits(g, homog, family_zc(k, 15))) return k;
     if (k >= 21 && k <= 23 && mplane && colx2wl_fits(g, homog, family_zc(k, 21))) return k;
     if constexpr (sizeof(T) == 4) {
-        if (k >= 24 && k <= 26 && whole && colqwl_fits(g, homog, family_zc(k, 24))) return k;
         if (k >= 18 && k <= 20 && whole && colq_fits(g, homog, family_zc(k, 18))) return k;
     }
-    if (k >= 12 && k <= 14 && whole && colp_fits<T>(g, family_zc(k, 12))) return k;
     if (k != 1 && whole) {
         const int zc = col_zc(g, k);
         return zc == 4 ? 6 : (zc == 2 ? 5 : 4);
@@ -791,19 +681,12 @@ hipError_t launch_step_impl(const Geo &g, const StepPtrs<T> &p, bool homog, cons
 {
     const int k = resolve_kernel<T>(g, homog, lc.kernel, lc.ncu);
     switch (k) {
-    case 2: return launch_march<T, EXACT, 1>(g, p, homog, lc.stream);
-    case 3: return launch_march<T, EXACT, 2>(g, p, homog, lc.stream);
     case 4: return launch_col<T, EXACT, 1>(g, p, homog, lc.stream);
     case 5: return launch_col<T, EXACT, 2>(g, p, homog, lc.stream);
     case 6: return launch_col<T, EXACT, 4>(g, p, homog, lc.stream);
-    case 7: return launch_plane<T, EXACT, 1>(g, p, homog, lc.stream);
-    case 8: return launch_plane<T, EXACT, 2>(g, p, homog, lc.stream);
     case 9: return launch_col<T, EXACT, 1, 1>(g, p, homog, lc.stream);
     case 10: return launch_col<T, EXACT, 2, 1>(g, p, homog, lc.stream);
     case 11: return launch_col<T, EXACT, 4, 1>(g, p, homog, lc.stream);
-    case 12: return launch_colp<T, EXACT, 1>(g, p, homog, lc.stream);
-    case 13: return launch_colp<T, EXACT, 2>(g, p, homog, lc.stream);
-    case 14: return launch_colp<T, EXACT, 4>(g, p, homog, lc.stream);
     case 15: return launch_colx2<T, EXACT, 1>(g, p, lc.stream);
     case 16: return launch_colx2<T, EXACT, 2>(g, p, lc.stream);
     case 17: return launch_colx2<T, EXACT, 4>(g, p, lc.stream);
@@ -817,9 +700,6 @@ hipError_t launch_step_impl(const Geo &g, const StepPtrs<T> &p, bool homog, cons
         case 18: return launch_colq<EXACT, 1>(g, p, lc.stream);
         case 19: return launch_colq<EXACT, 2>(g, p, lc.stream);
         case 20: return launch_colq<EXACT, 4>(g, p, lc.stream);
-        case 24: return launch_colq<EXACT, 1, true>(g, p, lc.stream);
-        case 25: return launch_colq<EXACT, 2, true>(g, p, lc.stream);
-        case 26: return launch_colq<EXACT, 4, true>(g, p, lc.stream);
         default: break;
         }
     }
@@ -857,7 +737,8 @@ inline uint32_t col_blocks(const Geo &g, const ColCfg &cc)
 // Ensemble step (rt6d_fast.hip): one launch per variant over its stretch of
 // the task list (tasks + off[v], cnt[v] tasks); persist_blocks = 0 runs one
 // block per task, else that many persistent blocks share the tasks through
-// the work counters ctr (kEnsCounters entries, cleared by the launch).
+// the work counters ctr (kEnsCounters entries: a set per step parity, each
+// step clearing the other parity's set; zero both before the first step).
 template <typename T>
 hipError_t launch_ensemble(const EnsMember<T> *mem, const EnsTask *tasks, const uint32_t off[kEnsVariants],
                            const uint32_t cnt[kEnsVariants], uint32_t parity, uint32_t persist_blocks, uint32_t *ctr,

@@ -32,6 +32,7 @@ FIELD = {"E": 0, "dE": 1, "meanfl": 2, "omega_i": 3, "omega_e": 4,
 LAYOUT_INTERIOR, LAYOUT_FACES, LAYOUT_HALO = 0, 1, 2
 OPT_EXACT, OPT_PATH, OPT_BLOCK, OPT_KERNEL, OPT_SYNC = 1, 2, 3, 4, 5
 TILES_WAVE_LOCAL, TILES_WAVE_LOCAL_1Z, TILES_PAIRS = 0, 1, 2
+EQ_LAUNCHES, EQ_TASKS = 1, 2
 Q_HOMOGENEOUS, Q_DEVICE_BYTES, Q_CELLS, Q_STEPS, Q_BYTES_PER_CELL, Q_KERNEL, Q_UNIFORM_XY = 1, 2, 3, 4, 5, 6, 7
 
 # every symbol include/rt6d.h declares
@@ -42,7 +43,7 @@ EXPORTS = ("rt6d_version", "rt6d_last_error", "rt6d_device_count", "rt6d_create"
            "rt6d_ensemble_create", "rt6d_ensemble_destroy", "rt6d_ensemble_add", "rt6d_ensemble_remove",
            "rt6d_ensemble_step", "rt6d_ensemble_set_schedule", "rt6d_ensemble_tune", "rt6d_ensemble_info",
            "rt6d_init_separable", "rt6d_branch_flips", "rt6d_ensemble_step_async", "rt6d_ensemble_wait",
-           "rt6d_set_forcing", "rt6d_diagnose_noalloc", "rt6d_ensemble_set_tiles")
+           "rt6d_set_forcing", "rt6d_diagnose_noalloc", "rt6d_ensemble_set_tiles", "rt6d_ensemble_query")
 
 
 class Diag(C.Structure):
@@ -109,6 +110,7 @@ def load_library(path: str | None = None):
         "rt6d_ensemble_tune": ([vp, C.c_int, ip, ip, ip], C.c_int),
         "rt6d_ensemble_set_tiles": ([vp, C.c_int], C.c_int),
         "rt6d_ensemble_info": ([vp, ip, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)], C.c_int),
+        "rt6d_ensemble_query": ([vp, C.c_int, C.POINTER(C.c_longlong)], C.c_int),
         "rt6d_init_separable": ([vp, dp, C.c_double, C.c_double, C.c_ulonglong, C.c_ulonglong], C.c_int),
         "rt6d_branch_flips": ([vp, vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)], C.c_int),
         "rt6d_ensemble_step_async": ([vp, C.c_int], C.c_int),
@@ -419,3 +421,9 @@ class Ensemble:
         _check(self.lib, self.lib.rt6d_ensemble_info(self._h, C.byref(m), C.byref(c), C.byref(s)),
                "rt6d_ensemble_info")
         return {"members": m.value, "cells": c.value, "steps": s.value}
+
+    def query(self, what: int) -> int:
+        """rt6d_ensemble_query: EQ_LAUNCHES (kernel launches per step) or EQ_TASKS."""
+        v = C.c_longlong()
+        _check(self.lib, self.lib.rt6d_ensemble_query(self._h, what, C.byref(v)), "rt6d_ensemble_query")
+        return v.value

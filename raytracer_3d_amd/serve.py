@@ -430,6 +430,7 @@ def ensemble_benchmark(members=96, steps=300, precision=rt6d.FP64, seed=4242, al
         for m in ms:
             ens.add(m)
         bpc, order, tiles = ens.tune(trial_steps=2)
+        launches = ens.query(rt6d.EQ_LAUNCHES)
         ens.step(2)
         t1 = time.perf_counter()
         dev_ms = ens.step(steps)
@@ -451,6 +452,7 @@ def ensemble_benchmark(members=96, steps=300, precision=rt6d.FP64, seed=4242, al
         w_alone += r.cells * steps
         m.close()
     out = {"members": members, "steps": steps, "cells": cells, "setup_s": setup, "schedule": [bpc, order], "tiles": tiles,
+           "launches_per_step": launches,
            "wall_s": wall, "device_ms": dev_ms, "value": cells * steps / wall,
            "one_at_a_time": w_alone / t_alone if t_alone else None, "one_at_a_time_members": len(sample),
            "one_at_a_time_wall_s": t_alone, "one_at_a_time_device_ms": d_alone,

@@ -24,15 +24,18 @@ WORKLOADS = {
     "packet": lambda: cases.packet_case(),
 }
 
+# RT6D_OPT_KERNEL options by name
+KERNELS = {"auto": 0, "percell": 1, "col1": 4, "col2": 5, "col4": 6, "coly1": 9, "coly2": 10, "coly4": 11,
+           "colx1": 15, "colx2": 16, "colx4": 17, "colq1": 18, "colq2": 19, "colq4": 20,
+           "colw1": 21, "colw2": 22, "colw4": 23}
+
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="gm", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--path", default="auto", choices=["auto", "general"])
-    ap.add_argument("--kernel", default="auto", choices=["auto", "percell", "march1", "march2", "col1", "col2", "col4", "plane1", "plane2",
-                                                        "coly1", "coly2", "coly4", "colp1", "colp2", "colp4",
-                                                        "colx1", "colx2", "colx4", "colq1", "colq2", "colq4", "colw1", "colw2", "colw4", "colqw1", "colqw2", "colqw4"])
+    ap.add_argument("--kernel", default="auto", choices=sorted(KERNELS, key=KERNELS.get))
     ap.add_argument("--exact", action="store_true")
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -43,11 +46,7 @@ def main(argv=None):
         print(json.dumps({"copy_bytes": a.copy_mib << 20, "copy_ms": ms}), flush=True)
     case = WORKLOADS[a.workload]()
     prec = rt6d.FP64 if a.precision == "fp64" else rt6d.FP32
-    kern = {"auto": 0, "percell": 1, "march1": 2, "march2": 3, "col1": 4, "col2": 5, "col4": 6, "plane1": 7,
-            "plane2": 8, "coly1": 9, "coly2": 10, "coly4": 11,
-            "colp1": 12, "colp2": 13, "colp4": 14, "colx1": 15, "colx2": 16, "colx4": 17,
-            "colq1": 18, "colq2": 19, "colq4": 20, "colw1": 21, "colw2": 22, "colw4": 23,
-            "colqw1": 24, "colqw2": 25, "colqw4": 26}[a.kernel]
+    kern = KERNELS[a.kernel]
     m = rt6d.Model.from_case(case, precision=prec, path=1 if a.path == "general" else 0, kernel=kern,
                              exact=a.exact)
     m.calc_parameter()

@@ -117,18 +117,25 @@ def test_generator_member_ensemble(precision, gpu_available):
         with rt6d.Ensemble(precision) as ens:
             for m in ms:
                 ens.add(m)
+            # every member of the generator's grids takes the same kernel
+            # variant under each tile choice: one launch per step
+            for tiles in (rt6d.TILES_PAIRS, rt6d.TILES_WAVE_LOCAL_1Z, rt6d.TILES_WAVE_LOCAL):
+                ens.set_tiles(tiles)
+                assert ens.query(rt6d.EQ_LAUNCHES) == 1, tiles
             ens.step(3)
+            # persistent blocks: a two-step graph and a single step, so that
+            # both parity sets of work counters are taken and cleared
             ens.set_schedule(4, 1)
-            ens.step(4)
-            assert ens.info()["steps"] == 7
+            ens.step(5)
+            assert ens.info()["steps"] == 8
             got = [m.download("E") for m in ms]
         for s, m, E in zip(specs, ms, got):
             with rt6d.Model.from_case(s, precision=precision) as solo:
                 solo.calc_parameter()
                 assert solo.cfl_dt(0.5) == m.dt
-                solo.integrate(7, dt=m.dt)
+                solo.integrate(8, dt=m.dt)
                 assert np.array_equal(E, solo.download("E")), f"{s.n} differs from its own run"
-        ref = _oracle_E(specs[-1], ms[-1].dt, 7)
+        ref = _oracle_E(specs[-1], ms[-1].dt, 8)
         err = rel_l2(got[-1], ref)
         assert err <= (TOL64 if precision == rt6d.FP64 else TOL32), err
     finally:
