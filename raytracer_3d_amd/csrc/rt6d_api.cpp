@@ -170,9 +170,9 @@ int check_ctx(rt6d_ctx *c)
 
 AxisCoef axis_coef(const rt6d_ctx *c);
 
-// The plane records of the homogeneous path for the coefficients' dt (queued
-// on the context's stream).
-int build_records(rt6d_ctx *c)
+// The plane records of the homogeneous path for step dt, the dt of the axis
+// coefficients (queued on the context's stream).
+int build_records(rt6d_ctx *c, double dt)
 {
     if (!c->d_hrec) return 0;
     const AxisCoef ax = axis_coef(c);
@@ -180,10 +180,10 @@ int build_records(rt6d_ctx *c)
         c->prec == RT6D_FP64
             ? launch_build_records<double>(c->g, (const double *)c->d_hx, (const double *)c->d_hy,
                                            (const double *)c->d_hz, (const double *)c->d_hm, (const double *)c->d_hl,
-                                           ax, (double *)c->d_hrec, c->stream)
+                                           ax, dt, (double *)c->d_hrec, c->stream)
             : launch_build_records<float>(c->g, (const float *)c->d_hx, (const float *)c->d_hy,
                                           (const float *)c->d_hz, (const float *)c->d_hm, (const float *)c->d_hl, ax,
-                                          (float *)c->d_hrec, c->stream);
+                                          dt, (float *)c->d_hrec, c->stream);
     HIPCHK(e);
     return 0;
 }
@@ -212,7 +212,7 @@ int ensure_coef(rt6d_ctx *c, double dt)
     }
     HIPCHK(hipMemcpyAsync(c->d_coef, h.data(), tot * sizeof(double), hipMemcpyHostToDevice, c->stream));
     int r;
-    if ((r = build_records(c))) return r;
+    if ((r = build_records(c, dt))) return r;
     HIPCHK(hipStreamSynchronize(c->stream));
     c->coef_dt = dt;
     return 0;
@@ -721,7 +721,7 @@ int rt6d_calc_parameter(rt6d_ctx *c)
         c->homog = bad == 0;
         if (c->homog) {
             if ((r = c->dalloc(&c->d_hrec, (size_t)kRec * c->n4 * es))) return r;
-            if ((r = build_records(c))) return r;
+            if ((r = build_records(c, c->coef_dt))) return r;
             HIPCHK(hipStreamSynchronize(c->stream));
             // no kernel of the homogeneous path reads the 6D coefficient
             // fields: free them (downloads recompute them on demand)

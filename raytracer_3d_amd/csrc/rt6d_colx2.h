@@ -425,18 +425,23 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
             }
             if (!ay) __builtin_amdgcn_s_setprio(0);
             if (az) {
-                const int zc = z0 + j;
-                const double dzc = sld(p.ax.dx[2], zc), rzc = sld(p.ax.rdx[2], zc);
-                dEa = div_term<T, EXACT>(dEa, FzA[j + 1], FzA[j], dzc, rzc);
-                dEb = div_term<T, EXACT>(dEb, FzB[j + 1], FzB[j], dzc, rzc);
+                if constexpr (EXACT) {
+                    const int zc = z0 + j;
+                    const double dzc = sld(p.ax.dx[2], zc), rzc = sld(p.ax.rdx[2], zc);
+                    dEa = div_term<T, EXACT>(dEa, FzA[j + 1], FzA[j], dzc, rzc);
+                    dEb = div_term<T, EXACT>(dEb, FzB[j + 1], FzB[j], dzc, rzc);
+                } else {
+                    // 1/dz of the cell from its record, already in T
+                    const T rz = rv(j, kRecRz);
+                    dEa = div_term_r<T>(dEa, FzA[j + 1], FzA[j], rz);
+                    dEb = div_term_r<T>(dEb, FzB[j + 1], FzB[j], rz);
+                }
             }
             if (am) {
-                const double dm = EXACT ? sld(p.ax.dx[5], m) : 1.0;
-                const double rm = sld(p.ax.cr[5], 2 * m + 1);
                 T FRa, FRb;
                 if constexpr (EXACT) {
                     const T vR = rvel(j, kRecMhi);
-                    const double cm = sld(p.ax.cr[5], 2 * m);
+                    const double dm = sld(p.ax.dx[5], m), cm = sld(p.ax.cr[5], 2 * m);
                     FRa = face_flux<T, EXACT>(vR, dtv, dm, cm, wm1[j].a, e.a, wp1[j].a, wp2[j].a);
                     FRb = face_flux<T, EXACT>(vR, dtv, dm, cm, wm1[j].b, e.b, wp1[j].b, wp2[j].b);
                 } else {
@@ -444,8 +449,15 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
                     FRa = sb_flux_c<T>(k, wm1[j].a, e.a, wp1[j].a, wp2[j].a);
                     FRb = sb_flux_c<T>(k, wm1[j].b, e.b, wp1[j].b, wp2[j].b);
                 }
-                dEa = div_term<T, EXACT>(dEa, FRa, FLm[j].a, dm, rm);
-                dEb = div_term<T, EXACT>(dEb, FRb, FLm[j].b, dm, rm);
+                if constexpr (EXACT) {
+                    const double dm = sld(p.ax.dx[5], m);
+                    dEa = div_term<T, EXACT>(dEa, FRa, FLm[j].a, dm, 0.0);
+                    dEb = div_term<T, EXACT>(dEb, FRb, FLm[j].b, dm, 0.0);
+                } else {
+                    const T rm = rv(j, kRecRm);
+                    dEa = div_term_r<T>(dEa, FRa, FLm[j].a, rm);
+                    dEb = div_term_r<T>(dEb, FRb, FLm[j].b, rm);
+                }
                 FLm[j] = Pair<T>{FRa, FRb};
             }
             const T lam = rv(j, kRecLam);
@@ -456,8 +468,9 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
                 en.a = e.a + p.dt * dEa;
                 en.b = e.b + p.dt * dEb;
             } else {
-                en.a = e.a + (T)p.dt * dEa;
-                en.b = e.b + (T)p.dt * dEb;
+                const T dtt = rv(j, kRecDt);   // the step's dt in T
+                en.a = e.a + dtt * dEa;
+                en.b = e.b + dtt * dEb;
             }
             if (!WL || live) bst2<T>(en, rout(m), oc, pb);
         }

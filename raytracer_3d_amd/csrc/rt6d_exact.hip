@@ -339,7 +339,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_homog_records(const Geo g, const T *__restrict__ vx3,
                                                        const T *__restrict__ vy3, const T *__restrict__ hz,
                                                        const T *__restrict__ hm, const T *__restrict__ hlam,
-                                                       const AxisCoef ax, T *__restrict__ rec)
+                                                       const AxisCoef ax, double dt, T *__restrict__ rec)
 {
     const uint32_t nz = g.n[2];
     const uint32_t n4 = g.ncells / ((uint32_t)g.n[0] * g.n[1]);
@@ -369,23 +369,26 @@ __global__ void __launch_bounds__(256) k_homog_records(const Geo g, const T *__r
     r[kRecLam] = hlam[P];
     if constexpr (sizeof(T) == 8) r[kRecFlags] = __builtin_bit_cast(double, (uint64_t)flags);
     else r[kRecFlags] = __builtin_bit_cast(float, flags);
-    for (int f = kRecFlags + 1; f < kRec; ++f) r[f] = T(0);
+    r[kRecRz] = (T)ax.rdx[2][z + 1];   // cell z+1 (1-based)
+    r[kRecRm] = (T)ax.rdx[5][m + 1];   // cell m+1 (1-based)
+    r[kRecDt] = (T)dt;
 }
 
 template <typename T>
 hipError_t launch_build_records(const Geo &g, const T *vx3, const T *vy3, const T *vz, const T *vm, const T *hlam,
-                                const AxisCoef &ax, T *rec, hipStream_t s)
+                                const AxisCoef &ax, double dt, T *rec, hipStream_t s)
 {
     const uint32_t n4 = g.ncells / ((uint32_t)g.n[0] * g.n[1]);
     hipLaunchKernelGGL(k_homog_records<T>, dim3(nblocks(n4, 256)), dim3(256), 0, s, g, vx3, vy3, vz, vm, hlam, ax,
-                       rec);
+                       dt, rec);
     return hipGetLastError();
 }
 template hipError_t launch_build_records<double>(const Geo &, const double *, const double *, const double *,
-                                                 const double *, const double *, const AxisCoef &, double *,
+                                                 const double *, const double *, const AxisCoef &, double, double *,
                                                  hipStream_t);
 template hipError_t launch_build_records<float>(const Geo &, const float *, const float *, const float *,
-                                                const float *, const float *, const AxisCoef &, float *, hipStream_t);
+                                                const float *, const float *, const AxisCoef &, double, float *,
+                                                hipStream_t);
 
 template hipError_t launch_build_homog<double>(const Geo &, const double *const[kAxes], const double *, double *,
                                                double *, double *, double *, double *, int *, hipStream_t);

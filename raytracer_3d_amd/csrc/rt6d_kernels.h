@@ -30,12 +30,15 @@ struct AxisCoef {
 //                   of the interior x and y faces (with the spacing of face
 //                   1, which is every face's where the spacing is uniform),
 //                   the z faces below and above and the m face above; the
-//                   velocities of the x and y boundary faces; lam; and bit
-//                   flags of which coefficient sets have v > 0
+//                   velocities of the x and y boundary faces; lam; bit
+//                   flags of which coefficient sets have v > 0; and, in the
+//                   record's precision, 1/dz of the plane's z cell, 1/dm of
+//                   its m cell and the step's dt
 constexpr int kRec = 24;
 enum RecField {
     kRecXi = 0, kRecYi = 3, kRecZlo = 6, kRecZhi = 9, kRecMhi = 12,   // (vp, vn, w) each
-    kRecVx0 = 15, kRecVxn = 16, kRecVy0 = 17, kRecVyn = 18, kRecLam = 19, kRecFlags = 20
+    kRecVx0 = 15, kRecVxn = 16, kRecVy0 = 17, kRecVyn = 18, kRecLam = 19, kRecFlags = 20,
+    kRecRz = 21, kRecRm = 22, kRecDt = 23
 };
 // flag bits of kRecFlags (in the low bits of that slot's bit pattern)
 enum RecFlag { kPosXi = 1, kPosYi = 2, kPosZlo = 4, kPosZhi = 8, kPosMhi = 16 };
@@ -180,6 +183,13 @@ __device__ __forceinline__ T div_term(T dE, T FR, T FL, double dxc, double rdxc)
 {
     if constexpr (EXACT) return dE - (FR - FL) / dxc;
     else return dE - (FR - FL) * (T)rdxc;
+}
+
+// the fast-arithmetic div_term with the reciprocal spacing already in T
+template <typename T>
+__device__ __forceinline__ T div_term_r(T dE, T FR, T FL, T rdxc)
+{
+    return dE - (FR - FL) * rdxc;
 }
 
 #include "rt6d_column.h"
@@ -791,7 +801,7 @@ hipError_t launch_build_homog(const Geo &g, const T *const vel[kAxes], const T *
 // step coefficients ax (dt/dx for the step's dt).
 template <typename T>
 hipError_t launch_build_records(const Geo &g, const T *vx3, const T *vy3, const T *vz, const T *vm, const T *hlam,
-                                const AxisCoef &ax, T *rec, hipStream_t s);
+                                const AxisCoef &ax, double dt, T *rec, hipStream_t s);
 // max over faces 1..n_d of |v_d| into vmax_bits[6] (as ordered 64-bit ints)
 hipError_t launch_vmax(const Geo &g, const double *const vel[kAxes], unsigned long long *vmax_bits,
                        hipStream_t s);
