@@ -157,11 +157,12 @@ def _step_kernel(precision, option, uxy):
     """Name of the kernel one step launches for a resolved kernel option
     (rt6d_query RT6D_Q_KERNEL), compiled for all advected axes active."""
     T = precision
-    wl = f"k_step_colx2<{T}, false, {{}}, true, true, {'true' if uxy else 'false'}>"
+    wl = f"k_step_colx2<{T}, false, {{}}, true, true, {'true' if uxy else 'false'}, false>"
     names = {
-        15: f"k_step_colx2<{T}, false, 1, true, false, false>", 16: f"k_step_colx2<{T}, false, 2, true, false, false>",
+        15: f"k_step_colx2<{T}, false, 1, true, false, false, false>",
+        16: f"k_step_colx2<{T}, false, 2, true, false, false, false>",
         21: wl.format(1), 22: wl.format(2),
-        18: "k_step_colq<false, 1, true, false>", 19: "k_step_colq<false, 2, true, false>",
+        18: "k_step_colq<false, 1, true>", 19: "k_step_colq<false, 2, true>",
         9: f"k_step_col<{T}, false, false, 1, true, 1>", 10: f"k_step_col<{T}, false, false, 2, true, 1>",
         4: f"k_step_col<{T}, false, true, 1, true, 0>", 5: f"k_step_col<{T}, false, true, 2, true, 0>",
     }

@@ -82,11 +82,30 @@ def _scaled(db, counter, kernel, copy_bytes):
     return v * scale, scale, v, n
 
 
+def counter_max(db, counter, substr):
+    """Per-dispatch maximum of a counter over its instances (one per XCD for
+    GRBM_GUI_ACTIVE: every XCD counts the dispatch's active cycles), averaged
+    over the dispatches."""
+    # pmc_events keeps one row per counter instance (counters_collection sums them)
+    rows = _rows(db, "select name, dispatch_id, counter_name, counter_value from pmc_events")
+    per = {}
+    for r in rows:
+        if r["counter_name"] != counter or substr not in r["name"]:
+            continue
+        per[r["dispatch_id"]] = max(per.get(r["dispatch_id"], 0.0), r["counter_value"])
+    vals = list(per.values())
+    return statistics.fmean(vals) if vals else None
+
+
 def busy_figures(db, kernel, cus):
-    """Unit activity of the kernel's dispatches from the busy pass."""
+    """Unit activity of the kernel's dispatches from the busy pass, as
+    rocprofv3's derived metrics define it: a unit's busy cycles over the
+    dispatch's active cycles, GRBM_GUI_ACTIVE taken as its maximum over the
+    XCDs (reduce(GRBM_GUI_ACTIVE, max)), not their sum."""
     g = {c: counter_avg(db, c, kernel)[0] for c in ("SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_WAVE_CYCLES",
                                                      "SQ_BUSY_CYCLES", "SQ_INSTS_SALU", "SQ_INSTS_SMEM",
-                                                     "GRBM_GUI_ACTIVE", "TA_BUSY_avr", "TD_TD_BUSY_sum")}
+                                                     "TA_BUSY_avr", "TD_TD_BUSY_sum")}
+    g["GRBM_GUI_ACTIVE"] = counter_max(db, "GRBM_GUI_ACTIVE", kernel)
     out = {"raw": g}
     gui = g["GRBM_GUI_ACTIVE"]
     if gui:
@@ -143,8 +162,9 @@ def step_summary(a) -> str:
     busy = busy_figures(a.busy, a.kernel, a.cus) if a.busy else None
     if busy:
         g = busy["raw"]
-        lines += ["", f"Unit activity per dispatch ({a.cus} CUs, GRBM_GUI_ACTIVE {g['GRBM_GUI_ACTIVE']:.0f} cycles):"]
-        for key, label in (("valu_busy", "VALU busy (SQ_ACTIVE_INST_VALU / CUs / GRBM_GUI_ACTIVE)"),
+        lines += ["", f"Unit activity per dispatch ({a.cus} CUs, GRBM_GUI_ACTIVE {g['GRBM_GUI_ACTIVE']:.0f} cycles, "
+                      "the maximum over the XCDs):"]
+        for key, label in (("valu_busy", "VALU busy (SQ_ACTIVE_INST_VALU / CUs / GRBM_GUI_ACTIVE, rocprofv3's VALUBusy)"),
                            ("ta_busy", "TA busy (TA_BUSY_avr / GRBM_GUI_ACTIVE)"),
                            ("td_busy", "TD busy (TD_TD_BUSY_sum / CUs / GRBM_GUI_ACTIVE)")):
             if key in busy:
