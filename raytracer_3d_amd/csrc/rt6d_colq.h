@@ -22,7 +22,6 @@ struct Quad {
 
 __device__ __forceinline__ Quad bld4(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
 {
-    using v4i = int __attribute__((ext_vector_type(4)));
     using v4f = float __attribute__((ext_vector_type(4)));
     const v4f f = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
     return Quad{{f.x, f.y, f.z, f.w}};

@@ -145,6 +145,8 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
 
     const int xy0 = x0 + nx * y0;
     const uint32_t oc = (uint32_t)xy0 * S;
+    // the wavefront holds a pair of row 1 or of row ny
+    const bool ybw = __ballot(y == 1 || y == ny) != 0;
     // x stencil pairs: (x-2, x-1) and (x+2, x+3); at the row ends the
     // zero-gradient ghosts repeat the pair's own edge cell
     const bool xfirst = x0 == 0, xlast = x0 + 2 >= nx;
@@ -222,15 +224,22 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
         return bld2<T>(rin(mu), oc, base0 + (uint32_t)(zsh[q] * (int)planeB) + moff(mu));
     };
     constexpr int LA = ZC >= 4 ? 3 : 2;   // own-plane look-ahead, as in the column kernel
-    Pair<T> wm1[ZC], w0[ZC], wp1[ZC], wp2[ZC], FLm[ZC];
+    // the m window E(m-1), E(m), E(m+1), E(m+2) of each z cell: four slots
+    // that the march renames rather than copies (RN: the uniform-spacing
+    // kernels and binary32; otherwise, and with four z cells per thread,
+    // where a fifth slot holds E(m+3), the slots are rotated by copies, which
+    // needs fewer registers)
+    Pair<T> win[5][ZC], FLm[ZC];
 #pragma unroll
     for (int j = 0; j < ZC; ++j) {
-        w0[j] = ldm(j, 1);
-        wp1[j] = ldm(j, 2);
-        if constexpr (LA == 3) wp2[j] = ldm(j, 3);
-        wm1[j] = w0[j];
+        win[1][j] = ldm(j, 1);
+        win[2][j] = ldm(j, 2);
+        if constexpr (LA == 3) win[3][j] = ldm(j, 3);
+        win[0][j] = win[1][j];
         FLm[j] = Pair<T>{T(0), T(0)};
     }
+    Pair<T>(&w0i)[ZC] = win[1];
+    Pair<T>(&wp1i)[ZC] = win[2];
     Pair<T> zq[4];
     auto ldhal = [&](int j, int mm) -> Pair<T> {
         Pair<T> h{T(0), T(0)};
@@ -246,15 +255,16 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
             // the product is subnormal)
             if constexpr (EXACT) {
                 const double d0 = sld(p.ax.dx[5], 0);
-                FLm[j] = Pair<T>{sb_flux_exact(vm0, p.dt, d0, w0[j].a, w0[j].a, w0[j].a, wp1[j].a),
-                                 sb_flux_exact(vm0, p.dt, d0, w0[j].b, w0[j].b, w0[j].b, wp1[j].b)};
+                FLm[j] = Pair<T>{sb_flux_exact(vm0, p.dt, d0, w0i[j].a, w0i[j].a, w0i[j].a, wp1i[j].a),
+                                 sb_flux_exact(vm0, p.dt, d0, w0i[j].b, w0i[j].b, w0i[j].b, wp1i[j].b)};
             } else {
-                FLm[j] = Pair<T>{vm0 * w0[j].a, vm0 * w0[j].b};
+                FLm[j] = Pair<T>{vm0 * w0i[j].a, vm0 * w0i[j].b};
             }
         }
     }
 
-    for (int m = 1; m <= nm; ++m) {
+    auto step = [&](int m, Pair<T>(&wm1)[ZC], Pair<T>(&w0)[ZC], Pair<T>(&wp1)[ZC], Pair<T>(&wp2)[ZC],
+                    Pair<T>(&wnx)[ZC]) {
         const uint32_t q20 = P0 + (uint32_t)(m - 1) * nzkl;
         // this step's coefficients: the plane records of the ZC cells, side by side
         const T *rp = p.hom.rec + (size_t)kRec * q20;
@@ -273,7 +283,6 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
             return k;
         };
         auto rvel = [&](int j, int at) -> T { return rv(j, at) + rv(j, at + 1); };   // vp + vn = v exactly
-        Pair<T> wnx[ZC];
 #pragma unroll
         for (int j = 0; j < ZC; ++j) {
             if constexpr (LA == 3) wnx[j] = ldm(j, m + 3);
@@ -349,8 +358,16 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
                     L = bld2<T>(rEm, oxl, pb);
                     R = bld2<T>(rEm, oxr, pb);
                 }
-                if (xfirst) L = Pair<T>{e.a, e.a};
-                if (xlast) R = Pair<T>{e.b, e.b};
+                if constexpr (EXACT) {
+                    if (xfirst) L = Pair<T>{e.a, e.a};
+                    if (xlast) R = Pair<T>{e.b, e.b};
+                } else {
+                    // the fast arithmetic replaces the boundary faces (F0 where
+                    // xfirst, F2 where xlast) by v*E of the edge cell, so only
+                    // the ghosts that face 1 reads are needed
+                    if (xfirst) L.b = e.a;
+                    if (xlast) R.a = e.b;
+                }
                 const T v0 = rv(j, kRecVx0), vb = rv(j, kRecVxn);
                 T F0, F1, F2;
                 if constexpr (EXACT) {
@@ -417,8 +434,12 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
                     FRa = sb_flux_c<T>(kr, a1.a, e.a, a3.a, a4.a);
                     FLb = sb_flux_c<T>(kl, a0.b, a1.b, e.b, a3.b);
                     FRb = sb_flux_c<T>(kr, a1.b, e.b, a3.b, a4.b);
-                    if (y == 1) { FLa = v0 * e.a; FLb = v0 * e.b; }
-                    if (y == ny) { FRa = vb * e.a; FRb = vb * e.b; }
+                    // boundary rows: only wavefronts that hold row 1 or row ny
+                    // (a wave-uniform branch)
+                    if (ybw) {
+                        if (y == 1) { FLa = v0 * e.a; FLb = v0 * e.b; }
+                        if (y == ny) { FRa = vb * e.a; FRb = vb * e.b; }
+                    }
                 }
                 dEa = div_term<T, EXACT>(dEa, FRa, FLa, dyr, ryy);
                 dEb = div_term<T, EXACT>(dEb, FRb, FLb, dyr, ryy);
@@ -474,18 +495,45 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
             }
             if (!WL || live) bst2<T>(en, rout(m), oc, pb);
         }
+    };
+    constexpr bool RN = LA == 2 && (UXY || sizeof(T) == 4);
+    if constexpr (RN) {
+        // four steps per trip, the window's slots renamed step by step
+        for (int m = 1; m <= nm; m += 4) {
+            step(m, win[0], win[1], win[2], win[3], win[4]);
+            if (m + 1 <= nm) step(m + 1, win[1], win[2], win[3], win[0], win[4]);
+            if (m + 2 <= nm) step(m + 2, win[2], win[3], win[0], win[1], win[4]);
+            if (m + 3 <= nm) step(m + 3, win[3], win[0], win[1], win[2], win[4]);
+        }
+    } else {
+        for (int m = 1; m <= nm; ++m) {
+            step(m, win[0], win[1], win[2], win[3], win[4]);
 #pragma unroll
-        for (int j = 0; j < ZC; ++j) {
-            wm1[j] = w0[j];
-            w0[j] = wp1[j];
-            wp1[j] = wp2[j];
-            if constexpr (LA == 3) wp2[j] = wnx[j];
+            for (int j = 0; j < ZC; ++j) {
+                win[0][j] = win[1][j];
+                win[1][j] = win[2][j];
+                win[2][j] = win[3][j];
+                if constexpr (LA == 3) win[3][j] = win[4][j];
+            }
         }
     }
 }
 
 template <typename T, bool EXACT, int ZC, bool FULL, bool WL = false, bool UXY = false, bool MP = false>
 __global__ void __launch_bounds__(256) k_step_colx2(const Geo g, const StepPtrs<T> p, const ColX2Cfg cc, uint32_t nbl)
+{
+    const uint32_t b = xcd_logical_block(blockIdx.x, nbl);
+    if (b >= nbl) return;
+    colx2_body<T, EXACT, ZC, FULL, WL, UXY, MP>(g, p, cc, b);
+}
+
+// The same kernel held to 128 VGPRs, four wavefronts per SIMD: the renamed
+// binary64 window of the uniform-spacing kernel takes one register more than
+// that otherwise (and then three wavefronts per SIMD), without spilling when
+// held to it.
+template <typename T, bool EXACT, int ZC, bool FULL, bool WL = false, bool UXY = false, bool MP = false>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+k_step_colx2_w4(const Geo g, const StepPtrs<T> p, const ColX2Cfg cc, uint32_t nbl)
 {
     const uint32_t b = xcd_logical_block(blockIdx.x, nbl);
     if (b >= nbl) return;

@@ -61,10 +61,15 @@ __device__ __forceinline__ void ens_task(const EnsMember<T> &M, uint32_t parity,
     }
 }
 
+// Held to 128 VGPRs (four wavefronts per SIMD) where the binary64
+// uniform-spacing wave-local body renames its m window, as the single-run
+// kernel is (launched one block per task: the persistent form would spill);
+// one wavefront per SIMD is the floor a 256-thread block sets anyway.
 template <typename T, bool HOMOG, int ZC, bool FULL, bool X2, bool Q, bool W, bool UXY, bool PERSIST>
-__global__ void __launch_bounds__(256) k_ens_step(const EnsMember<T> *__restrict__ mem,
-                                                  const EnsTask *__restrict__ tasks, uint32_t ntasks,
-                                                  uint32_t parity, uint32_t *ctr)
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 && W && UXY && ZC <= 2 && !PERSIST ? 4 : 1)))
+k_ens_step(const EnsMember<T> *__restrict__ mem, const EnsTask *__restrict__ tasks, uint32_t ntasks,
+           uint32_t parity, uint32_t *ctr)
 {
     if constexpr (!PERSIST) {
         const uint32_t t = xcd_logical_block(blockIdx.x, ntasks);

@@ -547,8 +547,12 @@ hipError_t launch_colx2(const Geo &g, const StepPtrs<T> &p, hipStream_t stream)
     // the wave-local kernels in fast arithmetic take uniform x and y spacings as scalars
     if constexpr (WL && !EXACT) {
         if ((g.uxy & 3) == 3 && is_full<true>(g)) {
-            hipLaunchKernelGGL((k_step_colx2<T, EXACT, ZC, true, WL, true>), dim3(nbh), dim3(256), 0, stream, g, p, cc,
-                               nbl);
+            if constexpr (sizeof(T) == 8 && ZC <= 2)
+                hipLaunchKernelGGL((k_step_colx2_w4<T, EXACT, ZC, true, WL, true>), dim3(nbh), dim3(256), 0, stream, g,
+                                   p, cc, nbl);
+            else
+                hipLaunchKernelGGL((k_step_colx2<T, EXACT, ZC, true, WL, true>), dim3(nbh), dim3(256), 0, stream, g, p,
+                                   cc, nbl);
             return hipGetLastError();
         }
     }
