@@ -1322,13 +1322,7 @@ int ens_rebuild_t(rt6d_ensemble *e)
             M.variant = (uint32_t)v;
             e->cur0[i] = a;
             if (x2 || q4 || wl) {
-                const uint32_t w = q4 ? 4u : 2u;   // x cells per thread
-                const uint32_t ngroups = (uint32_t)(c->g.n[0] / w) * c->g.n[1];
-                M.cx.nxyp = (int)(((ngroups + 63u) / 64u) * 64u);
-                M.cx.nzb = c->g.n[2] / zc;
-                M.cx.dxyp = make_fastdiv((uint32_t)M.cx.nxyp);
-                M.cx.dzb = make_fastdiv((uint32_t)M.cx.nzb);
-                M.cx.dxh = make_fastdiv((uint32_t)(c->g.n[0] / w));
+                M.cx = colx2_config(c->g, zc, q4 ? 4 : 2, false);   // x cells per thread: 4 or 2
                 const uint64_t nthreads = (uint64_t)M.cx.nxyp * M.cx.nzb * c->g.n[3] * c->g.n[4];
                 nbl.push_back((uint32_t)((nthreads + 255) / 256));
             } else {

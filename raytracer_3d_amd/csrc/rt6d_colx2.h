@@ -66,16 +66,57 @@ __device__ __forceinline__ void bst2<float>(Pair<float> v, __amdgpu_buffer_rsrc_
 }
 
 // Column configuration of the pair kernel: the padded plane counts pairs.
+// A plane's pairs are dealt to wavefronts wn at a time: 64, or (where asked
+// for and where that takes no more wavefronts) the whole rows that fit in 64
+// lanes, so that every wavefront starts at a row start (xal; also where 64 is
+// a multiple of nx/2).
 struct ColX2Cfg {
     FastDiv dxyp;  // division by the padded (pair, y) plane size
     FastDiv dzb;   // division by the number of z blocks
     FastDiv dxh;   // division by nx/2
-    int nxyp;      // (nx/2)*ny rounded up to a multiple of 64
+    int nxyp;      // 64 lanes per wavefront of the plane: 64 * ceil((nx/2)*ny / wn)
     int nzb;       // nz / ZC
+    int wn;        // pairs per wavefront
+    int xal;       // 1: wavefronts start at row starts (wn a multiple of nx/2)
 };
 
+// The pair (or quad) configuration of a grid: w x cells per thread, ZC z
+// cells; row-aligned wavefronts where asked for and free of extra wavefronts.
+inline ColX2Cfg colx2_config(const Geo &g, int zc, int w, bool rowal)
+{
+    const uint32_t nxh = (uint32_t)(g.n[0] / w), npairs = nxh * (uint32_t)g.n[1];
+    uint32_t wn = 64;
+    if (rowal && nxh <= 64) {
+        const uint32_t wr = (64u / nxh) * nxh;
+        if ((npairs + wr - 1) / wr <= (npairs + 63u) / 64u) wn = wr;
+    }
+    ColX2Cfg cc;
+    cc.wn = (int)wn;
+    cc.xal = wn % nxh == 0 ? 1 : 0;
+    cc.nxyp = (int)(((npairs + wn - 1) / wn) * 64u);
+    cc.nzb = g.n[2] / zc;
+    cc.dxyp = make_fastdiv((uint32_t)cc.nxyp);
+    cc.dzb = make_fastdiv((uint32_t)cc.nzb);
+    cc.dxh = make_fastdiv(nxh);
+    return cc;
+}
+
+// v of the lane below (lane i receives lane i-1's value; lane 0 receives 0)
+__device__ __forceinline__ float lane_shr1(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138 /* wave_shr:1 */,
+                                                                 0xf, 0xf, false));
+}
+__device__ __forceinline__ double lane_shr1(double v)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, 0x138, 0xf, 0xf, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), 0x138, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
 // Wave-local neighbour exchange (WL): each wavefront of the block keeps, per
-// z cell, 128 pair slots in LDS: H = nx pairs (two rows) before its 64 pairs,
+// z cell, 128 pair slots in LDS: H = nx pairs (two rows) before its wn pairs,
 // its own pairs, and H after (nx <= 32).  Every step a wave writes its own
 // pairs and loads the halo pairs once (one pair access by lanes 0..2H-1),
 // then reads the x and y neighbour pairs from the slots.  Only the wave's own
@@ -87,15 +128,28 @@ constexpr int kWLSlots = 128;
 // UXY: the x and y spacings are uniform (every face and cell of both axes
 // has the same dx), so that their coefficients are wave-uniform scalars
 // rather than per-lane values (fast arithmetic only).
-template <typename T, bool EXACT, int ZC, bool FULL, bool WL = false, bool UXY = false, bool MP = false>
+template <typename T, bool EXACT, int ZC, bool FULL, bool WL = false, bool UXY = false, bool MP = false,
+          bool XAL = false>
 __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, const ColX2Cfg &cc, uint32_t b)
 {
     static_assert(!(UXY && EXACT), "the exact arithmetic divides by the per-lane spacings");
+    static_assert(!XAL || (WL && !EXACT), "the x-face hand-over needs row-aligned wave-local lanes");
     const uint32_t t = b * blockDim.x + threadIdx.x;
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2], nk = g.n[3], nl = g.n[4], nm = g.n[5];
     const int nxy = nx * ny, nxh = nx >> 1;
     const uint32_t pl = __builtin_amdgcn_readfirstlane(fdiv(t, cc.dxyp));
-    const int xyp = (int)(t - pl * (uint32_t)cc.nxyp);
+    const int lane = (int)(threadIdx.x & 63u);   // nxyp and the block are whole wavefronts
+    // pairs per wavefront: fewer than 64 only where whole rows are dealt (XAL)
+    const int wn = XAL ? cc.wn : 64;
+    // the pair and the wavefront's first pair
+    int xyp, pw;
+    if constexpr (XAL) {
+        pw = (int)__builtin_amdgcn_readfirstlane((t - pl * (uint32_t)cc.nxyp) >> 6) * wn;
+        xyp = pw + lane;
+    } else {
+        xyp = (int)(t - pl * (uint32_t)cc.nxyp);
+        pw = xyp & ~63;
+    }
     const uint32_t kl = fdiv(pl, cc.dzb);
     const int zb = (int)(pl - kl * (uint32_t)cc.nzb);
     const int bl = (int)fdiv(kl, g.div[3]);
@@ -105,10 +159,13 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
     if constexpr (!WL) {
         if (xyp >= npairs) return;   // padding lanes (no barriers below)
     }
-    // WL: padding lanes stay to load halo pairs; they compute on the plane's
-    // last pair and store nothing
-    const bool live = xyp < npairs;
-    const int xyc = live ? xyp : npairs - 1;
+    // WL: padding lanes stay to load halo pairs; they compute on the
+    // wavefront's last pair and store nothing
+    const bool live = lane < wn && xyp < npairs;
+    // the wavefront's last pair (with 64 pairs per wavefront only the plane's
+    // last wavefront has padding lanes)
+    const int plast = XAL ? (pw + wn < npairs ? pw + wn : npairs) - 1 : npairs - 1;
+    const int xyc = live ? xyp : plast;
     const int y0 = (int)fdiv((uint32_t)xyc, cc.dxh);
     const int x0 = 2 * (xyc - y0 * nxh);   // 0-based first cell of the pair
     const int x = x0 + 1, y = y0 + 1;      // 1-based: the pair is (x, x+1)
@@ -191,15 +248,14 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
             zmir[q] = mir;
         }
     }
-    // WL slots: the wave's first pair, the halo pair this lane loads, and the
-    // slots of the x and y neighbour pairs
+    // WL slots: the halo pair this lane loads, and the slots of the x and y
+    // neighbour pairs
     using V2 = T __attribute__((ext_vector_type(2)));
     const int H = nx;                              // pairs in two rows
-    const int lane = (int)(threadIdx.x & 63u), wv = (int)(threadIdx.x >> 6);
-    const int pw = xyp & ~63;
+    const int wv = (int)(threadIdx.x >> 6);
     int hslot = 0, hp = -1;
     if (lane < H) { hp = pw - H + lane; hslot = lane; }
-    else if (lane < 2 * H) { hp = pw + 64 + (lane - H); hslot = 64 + lane; }
+    else if (lane < 2 * H) { hp = pw + wn + (lane - H); hslot = wn + lane; }
     const bool hval = WL && hp >= 0 && hp < npairs;
     uint32_t ohal = 0;
     if (hval) {
@@ -390,10 +446,20 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
                         k2 = sb_coef<T>(vi, (T)cx2);
                     }
                     F1 = sb_flux_c<T>(k1, L.b, e.a, e.b, R.a);
-                    F0 = sb_flux_c<T>(k0, L.a, L.b, e.a, e.b);
-                    F2 = sb_flux_c<T>(k2, e.a, e.b, R.a, R.b);
-                    if (xfirst) F0 = v0 * e.a;
-                    if (xlast) F2 = vb * e.b;
+                    if constexpr (XAL) {
+                        // face x-1 is face x+1 of the pair to the left, held by
+                        // the lane below, the wavefronts starting at row starts
+                        // (the same operands and coefficients, so the same bits)
+                        F2 = sb_flux_c<T>(k2, e.a, e.b, R.a, R.b);
+                        if (xlast) F2 = vb * e.b;
+                        F0 = lane_shr1(F2);
+                        if (xfirst) F0 = v0 * e.a;
+                    } else {
+                        F0 = sb_flux_c<T>(k0, L.a, L.b, e.a, e.b);
+                        F2 = sb_flux_c<T>(k2, e.a, e.b, R.a, R.b);
+                        if (xfirst) F0 = v0 * e.a;
+                        if (xlast) F2 = vb * e.b;
+                    }
                 }
                 dEa = div_term<T, EXACT>(dEa, F1, F0, dx1, rxa);
                 dEb = div_term<T, EXACT>(dEb, F2, F1, dx2, rxb);
@@ -519,23 +585,25 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
     }
 }
 
-template <typename T, bool EXACT, int ZC, bool FULL, bool WL = false, bool UXY = false, bool MP = false>
+template <typename T, bool EXACT, int ZC, bool FULL, bool WL = false, bool UXY = false, bool MP = false,
+          bool XAL = false>
 __global__ void __launch_bounds__(256) k_step_colx2(const Geo g, const StepPtrs<T> p, const ColX2Cfg cc, uint32_t nbl)
 {
     const uint32_t b = xcd_logical_block(blockIdx.x, nbl);
     if (b >= nbl) return;
-    colx2_body<T, EXACT, ZC, FULL, WL, UXY, MP>(g, p, cc, b);
+    colx2_body<T, EXACT, ZC, FULL, WL, UXY, MP, XAL>(g, p, cc, b);
 }
 
 // The same kernel held to 128 VGPRs, four wavefronts per SIMD: the renamed
 // binary64 window of the uniform-spacing kernel takes one register more than
 // that otherwise (and then three wavefronts per SIMD), without spilling when
 // held to it.
-template <typename T, bool EXACT, int ZC, bool FULL, bool WL = false, bool UXY = false, bool MP = false>
+template <typename T, bool EXACT, int ZC, bool FULL, bool WL = false, bool UXY = false, bool MP = false,
+          bool XAL = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 k_step_colx2_w4(const Geo g, const StepPtrs<T> p, const ColX2Cfg cc, uint32_t nbl)
 {
     const uint32_t b = xcd_logical_block(blockIdx.x, nbl);
     if (b >= nbl) return;
-    colx2_body<T, EXACT, ZC, FULL, WL, UXY, MP>(g, p, cc, b);
+    colx2_body<T, EXACT, ZC, FULL, WL, UXY, MP, XAL>(g, p, cc, b);
 }

@@ -524,13 +524,14 @@ inline bool colx2_fills(const Geo &g, int zc, int ncu)
 template <typename T, bool EXACT, int ZC, bool WL = false>
 hipError_t launch_colx2(const Geo &g, const StepPtrs<T> &p, hipStream_t stream)
 {
-    const uint32_t npairs = (uint32_t)(g.n[0] / 2) * g.n[1];
-    ColX2Cfg cc;
-    cc.nxyp = (int)(((npairs + 63u) / 64u) * 64u);
-    cc.nzb = g.n[2] / ZC;
-    cc.dxyp = make_fastdiv((uint32_t)cc.nxyp);
-    cc.dzb = make_fastdiv((uint32_t)cc.nzb);
-    cc.dxh = make_fastdiv((uint32_t)(g.n[0] / 2));
+    // the wave-local kernels in fast arithmetic take uniform x and y spacings
+    // as scalars; where the wavefronts start at row starts they also hand
+    // the x faces between pairs over from lane to lane.  Binary64 deals
+    // whole rows to wavefronts where that takes no more of them (at 24^3 x
+    // 16^3 0.2669 against 0.2725 ms); binary32 only where rows align anyway
+    // (the 60-pair wavefronts of 24^3 measured 0.1485 against 0.1464 ms).
+    const bool uxyw = WL && !EXACT && (g.uxy & 3) == 3 && is_full<true>(g);
+    const ColX2Cfg cc = colx2_config(g, ZC, 2, uxyw && sizeof(T) == 8 && ZC <= 2 && whole_fits32<T>(g));
     const uint64_t nthreads = (uint64_t)cc.nxyp * cc.nzb * g.n[3] * g.n[4];
     const uint32_t nbl = (uint32_t)((nthreads + 255) / 256);
     const uint32_t nbh = ((nbl + 7u) / 8u) * 8u;
@@ -544,15 +545,30 @@ hipError_t launch_colx2(const Geo &g, const StepPtrs<T> &p, hipStream_t stream)
                                p, cc, nbl);
         return hipGetLastError();
     }
-    // the wave-local kernels in fast arithmetic take uniform x and y spacings as scalars
-    if constexpr (WL && !EXACT) {
-        if ((g.uxy & 3) == 3 && is_full<true>(g)) {
-            if constexpr (sizeof(T) == 8 && ZC <= 2)
-                hipLaunchKernelGGL((k_step_colx2_w4<T, EXACT, ZC, true, WL, true>), dim3(nbh), dim3(256), 0, stream, g,
-                                   p, cc, nbl);
-            else
-                hipLaunchKernelGGL((k_step_colx2<T, EXACT, ZC, true, WL, true>), dim3(nbh), dim3(256), 0, stream, g, p,
-                                   cc, nbl);
+    if constexpr (WL && !EXACT && ZC <= 2) {
+        if (uxyw) {
+            if constexpr (sizeof(T) == 8) {
+                if (cc.xal)
+                    hipLaunchKernelGGL((k_step_colx2_w4<T, EXACT, ZC, true, WL, true, false, true>), dim3(nbh),
+                                       dim3(256), 0, stream, g, p, cc, nbl);
+                else
+                    hipLaunchKernelGGL((k_step_colx2_w4<T, EXACT, ZC, true, WL, true>), dim3(nbh), dim3(256), 0,
+                                       stream, g, p, cc, nbl);
+            } else {
+                if (cc.xal)
+                    hipLaunchKernelGGL((k_step_colx2<T, EXACT, ZC, true, WL, true, false, true>), dim3(nbh), dim3(256),
+                                       0, stream, g, p, cc, nbl);
+                else
+                    hipLaunchKernelGGL((k_step_colx2<T, EXACT, ZC, true, WL, true>), dim3(nbh), dim3(256), 0, stream,
+                                       g, p, cc, nbl);
+            }
+            return hipGetLastError();
+        }
+    }
+    if constexpr (WL && !EXACT && ZC > 2) {
+        if (uxyw) {
+            hipLaunchKernelGGL((k_step_colx2<T, EXACT, ZC, true, WL, true>), dim3(nbh), dim3(256), 0, stream, g, p, cc,
+                               nbl);
             return hipGetLastError();
         }
     }
@@ -585,13 +601,7 @@ inline bool colq_fills(const Geo &g, int zc, int ncu)
 template <bool EXACT, int ZC>
 hipError_t launch_colq(const Geo &g, const StepPtrs<float> &p, hipStream_t stream)
 {
-    const uint32_t nquads = (uint32_t)(g.n[0] / 4) * g.n[1];
-    ColQCfg cc;
-    cc.nxyp = (int)(((nquads + 63u) / 64u) * 64u);
-    cc.nzb = g.n[2] / ZC;
-    cc.dxyp = make_fastdiv((uint32_t)cc.nxyp);
-    cc.dzb = make_fastdiv((uint32_t)cc.nzb);
-    cc.dxh = make_fastdiv((uint32_t)(g.n[0] / 4));
+    const ColQCfg cc = colx2_config(g, ZC, 4, false);
     const uint64_t nthreads = (uint64_t)cc.nxyp * cc.nzb * g.n[3] * g.n[4];
     const uint32_t nbl = (uint32_t)((nthreads + 255) / 256);
     const uint32_t nbh = ((nbl + 7u) / 8u) * 8u;
