@@ -153,14 +153,17 @@ def _traffic(workload, tag):
         return None
 
 
-def _step_kernel(precision, option, uxy):
+def _step_kernel(precision, option, uxy, handover=False):
     """Name of the kernel one step launches for a resolved kernel option
-    (rt6d_query RT6D_Q_KERNEL), compiled for all advected axes active."""
+    (rt6d_query RT6D_Q_KERNEL), compiled for all advected axes active;
+    handover: RT6D_Q_X_HANDOVER (the row-aligned wave-local kernel)."""
     T = precision
-    wl = f"k_step_colx2<{T}, false, {{}}, true, true, {'true' if uxy else 'false'}, false>"
+    # binary64's uniform-spacing wave-local kernels are the 128-VGPR entry
+    k = "k_step_colx2_w4" if (uxy and T == "double") else "k_step_colx2"
+    wl = f"{k}<{T}, false, {{}}, true, true, {'true' if uxy else 'false'}, false, {'true' if handover else 'false'}>"
     names = {
-        15: f"k_step_colx2<{T}, false, 1, true, false, false, false>",
-        16: f"k_step_colx2<{T}, false, 2, true, false, false, false>",
+        15: f"k_step_colx2<{T}, false, 1, true, false, false, false, false>",
+        16: f"k_step_colx2<{T}, false, 2, true, false, false, false, false>",
         21: wl.format(1), 22: wl.format(2),
         18: "k_step_colq<false, 1, true>", 19: "k_step_colq<false, 2, true>",
         9: f"k_step_col<{T}, false, false, 1, true, 1>", 10: f"k_step_col<{T}, false, false, 2, true, 1>",
@@ -235,6 +238,7 @@ def main():
             "homogeneous": m.homogeneous,
             "kernel_option": m.kernel,
             "uniform_xy": m.query(rt6d.Q_UNIFORM_XY),
+            "x_handover": m.query(rt6d.Q_X_HANDOVER),
             "dt": dt,
             "settle_steps": settle,
         }
@@ -296,7 +300,7 @@ def main():
             extras["serving"] = serve.serve_benchmark(40.0, args.serve_seconds, 200.0, baseline_seconds=10.0,
                                                       log=_log)
 
-    kname = _step_kernel("double", main_res["kernel_option"], main_res["uniform_xy"] == 3)
+    kname = _step_kernel("double", main_res["kernel_option"], main_res["uniform_xy"] == 3, main_res["x_handover"] == 1)
     ttag = "fp64_auto" if main_res["homogeneous"] else "fp64_general"
     achieved = main_res["achieved_GBs"]
     tr = _traffic(workload, ttag)

@@ -115,6 +115,9 @@ extern "C" {
                                 fallback for a grid the option does not fit (the dispatch's
                                 own resolution; valid after calc_parameter, which decides
                                 the homogeneous path) */
+#define RT6D_Q_X_HANDOVER 8  /* 1 if that kernel is the wave-local pair kernel whose wavefronts
+                                start at row starts and pass the x face between two pairs
+                                from lane to lane (fast mode, uniform x/y spacing) */
 
 typedef struct rt6d_ctx rt6d_ctx;
 

@@ -1229,6 +1229,15 @@ int rt6d_query(rt6d_ctx *c, int what, long long *value)
         *value = c->prec == RT6D_FP64 ? resolve_kernel<double>(c->g, c->homog, c->opt_kernel, c->ncu)
                                       : resolve_kernel<float>(c->g, c->homog, c->opt_kernel, c->ncu);
         return 0;
+    case RT6D_Q_X_HANDOVER: {
+        const int k = c->prec == RT6D_FP64 ? resolve_kernel<double>(c->g, c->homog, c->opt_kernel, c->ncu)
+                                           : resolve_kernel<float>(c->g, c->homog, c->opt_kernel, c->ncu);
+        const int zc = k == 21 ? 1 : 2;
+        const bool wl = c->homog && (k == 21 || k == 22);
+        *value = wl && (c->prec == RT6D_FP64 ? colx2_hands_over<double>(c->g, c->opt_exact, zc)
+                                             : colx2_hands_over<float>(c->g, c->opt_exact, zc));
+        return 0;
+    }
     case RT6D_Q_BYTES_PER_CELL: {
         // E read + E write, plus the streamed coefficient fields on the general path
         double b = 2.0 * c->esz;

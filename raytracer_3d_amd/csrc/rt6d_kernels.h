@@ -521,6 +521,22 @@ inline bool colx2_fills(const Geo &g, int zc, int ncu)
     return 2 * waves >= 5ull * 4u * (uint64_t)(ncu > 0 ? ncu : 256);
 }
 
+// Whether the wave-local pair kernel with ZC z cells per thread hands the x
+// faces between pairs from lane to lane (the XAL kernels): fast arithmetic
+// with uniform x/y spacings on a full grid within 32-bit offsets, ZC <= 2,
+// and wavefronts that start at row starts, which binary64 arranges by
+// dealing whole rows where that takes no more wavefronts.
+template <typename T>
+inline bool colx2_rowal(const Geo &g, bool exact, int zc)
+{
+    return !exact && (g.uxy & 3) == 3 && is_full<true>(g) && zc <= 2 && whole_fits32<T>(g);
+}
+template <typename T>
+inline bool colx2_hands_over(const Geo &g, bool exact, int zc)
+{
+    return colx2_rowal<T>(g, exact, zc) && colx2_config(g, zc, 2, sizeof(T) == 8).xal;
+}
+
 template <typename T, bool EXACT, int ZC, bool WL = false>
 hipError_t launch_colx2(const Geo &g, const StepPtrs<T> &p, hipStream_t stream)
 {
@@ -531,7 +547,7 @@ hipError_t launch_colx2(const Geo &g, const StepPtrs<T> &p, hipStream_t stream)
     // 16^3 0.2669 against 0.2725 ms); binary32 only where rows align anyway
     // (the 60-pair wavefronts of 24^3 measured 0.1485 against 0.1464 ms).
     const bool uxyw = WL && !EXACT && (g.uxy & 3) == 3 && is_full<true>(g);
-    const ColX2Cfg cc = colx2_config(g, ZC, 2, uxyw && sizeof(T) == 8 && ZC <= 2 && whole_fits32<T>(g));
+    const ColX2Cfg cc = colx2_config(g, ZC, 2, WL && sizeof(T) == 8 && colx2_rowal<T>(g, EXACT, ZC));
     const uint64_t nthreads = (uint64_t)cc.nxyp * cc.nzb * g.n[3] * g.n[4];
     const uint32_t nbl = (uint32_t)((nthreads + 255) / 256);
     const uint32_t nbh = ((nbl + 7u) / 8u) * 8u;

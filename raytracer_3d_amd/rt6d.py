@@ -34,6 +34,7 @@ OPT_EXACT, OPT_PATH, OPT_BLOCK, OPT_KERNEL, OPT_SYNC = 1, 2, 3, 4, 5
 TILES_WAVE_LOCAL, TILES_WAVE_LOCAL_1Z, TILES_PAIRS = 0, 1, 2
 EQ_LAUNCHES, EQ_TASKS = 1, 2
 Q_HOMOGENEOUS, Q_DEVICE_BYTES, Q_CELLS, Q_STEPS, Q_BYTES_PER_CELL, Q_KERNEL, Q_UNIFORM_XY = 1, 2, 3, 4, 5, 6, 7
+Q_X_HANDOVER = 8
 
 # every symbol include/rt6d.h declares
 EXPORTS = ("rt6d_version", "rt6d_last_error", "rt6d_device_count", "rt6d_create", "rt6d_destroy",

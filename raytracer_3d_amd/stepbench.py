@@ -58,7 +58,7 @@ def main(argv=None):
     t1 = time.perf_counter()
     out = {
         "workload": a.workload, "n": list(case.n), "precision": a.precision, "path": a.path, "kernel": a.kernel,
-        "kernel_option": m.kernel,
+        "kernel_option": m.kernel, "x_handover": m.query(rt6d.Q_X_HANDOVER),
         "exact": a.exact, "homogeneous": m.homogeneous, "steps": a.steps, "launch_ms": ms / a.steps,
         "wall_ms_per_step": 1e3 * (t1 - t0) / a.steps, "bytes_per_cell": m.bytes_per_cell,
         "cell_updates_per_s": case.cells * a.steps / (t1 - t0),

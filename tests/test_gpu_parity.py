@@ -205,14 +205,19 @@ def test_fp32_quad_kernel_is_bitwise_the_column_kernel(n, gpu_available):
 
 @pytest.mark.parametrize("precision", [rt6d.FP64, rt6d.FP32], ids=["fp64", "fp32"])
 @pytest.mark.parametrize("n", [(24, 24, 4, 4, 4, 4), (32, 8, 4, 4, 3, 4), (12, 5, 6, 4, 3, 5), (2, 7, 4, 3, 4, 4),
-                               (30, 3, 4, 4, 4, 4), (16, 1, 4, 4, 4, 4), (4, 9, 4, 3, 3, 4), (64, 3, 4, 3, 3, 4)],
+                               (30, 3, 4, 4, 4, 4), (16, 1, 4, 4, 4, 4), (4, 9, 4, 3, 3, 4), (64, 3, 4, 3, 3, 4),
+                               (22, 22, 4, 4, 3, 4), (20, 13, 4, 3, 3, 4)],
                          ids=lambda n: "x".join(map(str, n)))
 def test_wave_local_pair_kernel_is_bitwise_the_column_kernel(n, precision, gpu_available):
     """The pair kernel with x and y neighbours exchanged through wave-local
     LDS slots (options 21/22/23) gives the column kernel's values bit for
     bit: halos cut by the plane's ends, ragged last waves (padding lanes that
     load halo pairs), a single pair per row, inactive y, nx up to 32 (and
-    wider rows, which fall back to the column kernel)."""
+    wider rows, which fall back to the column kernel).  Where wavefronts
+    start at row starts (64 a multiple of nx/2, or in binary64 whole rows
+    dealt to wavefronts: 24, 12, 30, 20 wide) the x face between two pairs
+    passes from lane to lane; at 22 x 22 whole rows would take one wavefront
+    more per plane, so it is computed by both lanes."""
     case = cases.packet_case(n=n, name="wave_local")
     outs = {}
     for kernel in (5, 21, 22, 23):
@@ -263,6 +268,24 @@ def test_auto_kernel_choice(n, precision, path, expected, gpu_available):
     with rt6d.Model.from_case(case, precision=precision, kernel=4) as m:
         m.calc_parameter()
         assert m.kernel == 4
+
+
+@pytest.mark.parametrize("n, precision, exact, kernel, expected", [
+    ((24, 24, 24, 16, 16, 16), rt6d.FP64, False, 0, 1),   # whole rows dealt: 5 rows of 12 pairs per wavefront
+    ((24, 24, 24, 16, 16, 16), rt6d.FP32, False, 0, 0),   # binary32 keeps 64 pairs per wavefront
+    ((32, 32, 32, 16, 16, 16), rt6d.FP32, False, 0, 1),   # 64 pairs are 4 rows anyway
+    ((24, 24, 24, 16, 16, 16), rt6d.FP64, True, 0, 0),    # exact mode computes every face
+    ((24, 24, 24, 16, 16, 16), rt6d.FP64, False, 16, 0),  # plain pairs
+    ((22, 22, 24, 8, 8, 8), rt6d.FP64, False, 22, 0),     # whole rows would add a wavefront per plane
+], ids=["fp64-24", "fp32-24", "fp32-32", "exact", "pairs", "fp64-22"])
+def test_x_face_handover_query(n, precision, exact, kernel, expected, gpu_available):
+    """RT6D_Q_X_HANDOVER: the wave-local pair kernel passes the x face between
+    two pairs from lane to lane only where its wavefronts start at row starts
+    (the results are those of the column kernel either way)."""
+    case = cases.member_spec(3, 2, n=n)
+    with rt6d.Model.from_case(case, precision=precision, exact=exact, kernel=kernel) as m:
+        m.calc_parameter()
+        assert m.query(rt6d.Q_X_HANDOVER) == expected
 
 
 @pytest.mark.parametrize("n, path, option, expected", [
