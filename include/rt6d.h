@@ -317,6 +317,10 @@ int rt6d_ensemble_info(rt6d_ensemble *ens, int *members, long long *cells, long 
                                 among the members, nothing else (members whose grids take
                                 the same variant share one launch) */
 #define RT6D_EQ_TASKS 2      /* (member, block) tasks in the work list */
+#define RT6D_EQ_X_HANDOVER 3 /* members stepped by the wave-local pair kernel with whole rows per
+                                wavefront and the x faces passed between lanes (binary64, where
+                                every wave-local member with uniform spacings can take it, so
+                                that they keep one launch) */
 int rt6d_ensemble_query(rt6d_ensemble *ens, int what, long long *value);
 
 #ifdef __cplusplus

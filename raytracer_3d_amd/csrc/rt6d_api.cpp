@@ -1282,6 +1282,7 @@ struct rt6d_ensemble {
     size_t task_cap = 0;
     uint32_t ntasks = 0;
     uint32_t off[kEnsVariants] = {0}, cnt[kEnsVariants] = {0};
+    long long nxal = 0;               // members whose step passes x faces between lanes
     uint32_t *d_ctr = nullptr;
     bool dirty = true;
     bool pending = false;             // steps enqueued, not yet waited for
@@ -1304,6 +1305,8 @@ int ens_rebuild_t(rt6d_ensemble *e)
     std::vector<EnsMember<T>> mem;
     std::vector<uint32_t> nbl;
     std::vector<int> var;
+    std::vector<std::pair<size_t, int>> rowal_cand;   // (member, z cells per thread)
+    bool rowal_all = true;
     for (size_t i = 0; i < e->slot.size(); ++i) {
         EnsMember<T> M{};
         rt6d_ctx *c = e->slot[i];
@@ -1326,12 +1329,19 @@ int ens_rebuild_t(rt6d_ensemble *e)
             const bool uxy = wl && full && (c->g.uxy & 3) == 3;
             const bool q4 = !wl && sizeof(T) == 4 && zc == 2 && colq_fits(c->g, c->homog, 2);
             const bool x2 = !wl && !q4 && colx2_fits(c->g, c->homog, zc);
+            if (x2 || q4 || wl) M.cx = colx2_config(c->g, zc, q4 ? 4 : 2, false);   // x cells per thread: 4 or 2
+            // binary64 wave-local members with uniform spacings that can deal
+            // whole rows to wavefronts without one more of them (decided for
+            // the whole ensemble below)
+            if (uxy && sizeof(T) == 8 && colx2_rowal<T>(c->g, false, zc)) {
+                rowal_cand.push_back({i, zc});
+                rowal_all = rowal_all && colx2_config(c->g, zc, 2, true).xal;
+            }
             v = (c->homog ? 1 : 0) | (zc == 2 ? 2 : 0) | (full ? 4 : 0) | (x2 ? 8 : 0) | (q4 ? 16 : 0) |
                 (wl ? 32 : 0) | (uxy ? 64 : 0);
             M.variant = (uint32_t)v;
             e->cur0[i] = a;
             if (x2 || q4 || wl) {
-                M.cx = colx2_config(c->g, zc, q4 ? 4 : 2, false);   // x cells per thread: 4 or 2
                 const uint64_t nthreads = (uint64_t)M.cx.nxyp * M.cx.nzb * c->g.n[3] * c->g.n[4];
                 nbl.push_back((uint32_t)((nthreads + 255) / 256));
             } else {
@@ -1342,6 +1352,21 @@ int ens_rebuild_t(rt6d_ensemble *e)
         }
         var.push_back(v);
         mem.push_back(M);
+    }
+    // Whole rows per wavefront with the x faces passed between lanes (variant
+    // bit 7), as the single-run kernel, where every such member can take it:
+    // one member that cannot would split the step into two launches.
+    e->nxal = rowal_all ? (long long)rowal_cand.size() : 0;
+    if (rowal_all) {
+        for (const auto &[i, zc] : rowal_cand) {
+            const rt6d_ctx *c = e->slot[i];
+            EnsMember<T> &M = mem[i];
+            M.cx = colx2_config(c->g, zc, 2, true);
+            var[i] |= 128;
+            M.variant = (uint32_t)var[i];
+            const uint64_t nthreads = (uint64_t)M.cx.nxyp * M.cx.nzb * c->g.n[3] * c->g.n[4];
+            nbl[i] = (uint32_t)((nthreads + 255) / 256);
+        }
     }
     // the work list, grouped by variant; within a variant member by member or interleaved
     std::vector<EnsTask> tasks;
@@ -1686,6 +1711,7 @@ int rt6d_ensemble_query(rt6d_ensemble *e, int what, long long *value)
         return 0;
     }
     case RT6D_EQ_TASKS: *value = (long long)e->ntasks; return 0;
+    case RT6D_EQ_X_HANDOVER: *value = e->nxal; return 0;
     default: return fail(RT6D_EINVAL, "rt6d_ensemble_query: unknown query %d", what);
     }
 }

@@ -35,7 +35,9 @@ struct EnsMember {
                         // divisible by 4, two z cells per thread); bit 5: two x cells
                         // per thread with the wave-local neighbour exchange (homogeneous,
                         // nx even and <= 32); bit 6: with bit 5 and all axes active,
-                        // uniform x and y spacings taken as scalars
+                        // uniform x and y spacings taken as scalars; bit 7: with bit 6
+                        // (binary64), whole rows per wavefront and the x faces between
+                        // pairs passed from lane to lane
     uint32_t pad;
 };
 
@@ -43,15 +45,15 @@ struct EnsTask {
     uint32_t member, block;
 };
 
-constexpr int kEnsVariants = 128;
+constexpr int kEnsVariants = 256;
 constexpr int kEnsCounterSet = kEnsVariants * 8;   // one work counter per variant and XCD
 constexpr int kEnsCounters = 2 * kEnsCounterSet;   // a set per step parity
 
-template <typename T, bool HOMOG, int ZC, bool FULL, bool X2, bool Q, bool W, bool UXY>
+template <typename T, bool HOMOG, int ZC, bool FULL, bool X2, bool Q, bool W, bool UXY, bool XAL>
 __device__ __forceinline__ void ens_task(const EnsMember<T> &M, uint32_t parity, uint32_t block)
 {
     if constexpr (W) {
-        colx2_body<T, false, ZC, FULL, true, UXY>(M.g, M.p[parity & 1u], M.cx, block);
+        colx2_body<T, false, ZC, FULL, true, UXY, false, XAL>(M.g, M.p[parity & 1u], M.cx, block);
     } else if constexpr (Q) {
         if constexpr (sizeof(T) == 4) colq_body<false, ZC, FULL>(M.g, M.p[parity & 1u], M.cx, block);
     } else if constexpr (X2) {
@@ -65,7 +67,7 @@ __device__ __forceinline__ void ens_task(const EnsMember<T> &M, uint32_t parity,
 // uniform-spacing wave-local body renames its m window, as the single-run
 // kernel is (launched one block per task: the persistent form would spill);
 // one wavefront per SIMD is the floor a 256-thread block sets anyway.
-template <typename T, bool HOMOG, int ZC, bool FULL, bool X2, bool Q, bool W, bool UXY, bool PERSIST>
+template <typename T, bool HOMOG, int ZC, bool FULL, bool X2, bool Q, bool W, bool UXY, bool XAL, bool PERSIST>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 && W && UXY && ZC <= 2 && !PERSIST ? 4 : 1)))
 k_ens_step(const EnsMember<T> *__restrict__ mem, const EnsTask *__restrict__ tasks, uint32_t ntasks,
@@ -75,7 +77,7 @@ k_ens_step(const EnsMember<T> *__restrict__ mem, const EnsTask *__restrict__ tas
         const uint32_t t = xcd_logical_block(blockIdx.x, ntasks);
         if (t >= ntasks) return;   // uniform over the block
         const EnsTask tk = tasks[t];
-        ens_task<T, HOMOG, ZC, FULL, X2, Q, W, UXY>(mem[tk.member], parity, tk.block);
+        ens_task<T, HOMOG, ZC, FULL, X2, Q, W, UXY, XAL>(mem[tk.member], parity, tk.block);
     } else {
         __shared__ uint32_t s_task;
         // ctr: this variant's counters in the set of parity 0
@@ -92,7 +94,7 @@ k_ens_step(const EnsMember<T> *__restrict__ mem, const EnsTask *__restrict__ tas
             __syncthreads();   // every thread has read s_task before thread 0 writes it again
             if (t >= t1) break;
             const EnsTask tk = tasks[t];
-            ens_task<T, HOMOG, ZC, FULL, X2, Q, W, UXY>(mem[tk.member], parity, tk.block);
+            ens_task<T, HOMOG, ZC, FULL, X2, Q, W, UXY, XAL>(mem[tk.member], parity, tk.block);
         }
     }
 }

@@ -48,18 +48,18 @@ hipError_t launch_diag_fast(const Geo &g, const StepPtrs<T> &p, bool homog, cons
 }
 
 template <typename T, bool HOMOG, int ZC, bool FULL, bool X2 = false, bool Q = false, bool W = false,
-          bool UXY = false>
+          bool UXY = false, bool XAL = false>
 static void ens_launch_variant(const EnsMember<T> *mem, const EnsTask *tasks, uint32_t n, uint32_t parity,
                                uint32_t persist_blocks, uint32_t *ctr, hipStream_t s)
 {
     if (persist_blocks) {
         uint32_t nb = persist_blocks < n ? persist_blocks : n;
         nb = ((nb + 7u) / 8u) * 8u;
-        hipLaunchKernelGGL((k_ens_step<T, HOMOG, ZC, FULL, X2, Q, W, UXY, true>), dim3(nb), dim3(256), 0, s, mem, tasks,
+        hipLaunchKernelGGL((k_ens_step<T, HOMOG, ZC, FULL, X2, Q, W, UXY, XAL, true>), dim3(nb), dim3(256), 0, s, mem, tasks,
                            n, parity, ctr);
     } else {
         const uint32_t nb = ((n + 7u) / 8u) * 8u;
-        hipLaunchKernelGGL((k_ens_step<T, HOMOG, ZC, FULL, X2, Q, W, UXY, false>), dim3(nb), dim3(256), 0, s, mem,
+        hipLaunchKernelGGL((k_ens_step<T, HOMOG, ZC, FULL, X2, Q, W, UXY, XAL, false>), dim3(nb), dim3(256), 0, s, mem,
                            tasks, n, parity, ctr);
     }
 }
@@ -107,6 +107,17 @@ hipError_t launch_ensemble(const EnsMember<T> *mem, const EnsTask *tasks, const 
             break;
         case 103:
             ens_launch_variant<T, true, 2, true, false, false, true, true>(mem, tv, cnt[v], parity, persist_blocks, cv, s);
+            break;
+        // the same in binary64 with whole rows per wavefront and the x faces passed between lanes (bit 7)
+        case 229:
+            if constexpr (sizeof(T) == 8)
+                ens_launch_variant<T, true, 1, true, false, false, true, true, true>(mem, tv, cnt[v], parity,
+                                                                                    persist_blocks, cv, s);
+            break;
+        case 231:
+            if constexpr (sizeof(T) == 8)
+                ens_launch_variant<T, true, 2, true, false, false, true, true, true>(mem, tv, cnt[v], parity,
+                                                                                    persist_blocks, cv, s);
             break;
         default: break;   // not produced by the host
         }

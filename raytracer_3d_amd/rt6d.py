@@ -32,7 +32,7 @@ FIELD = {"E": 0, "dE": 1, "meanfl": 2, "omega_i": 3, "omega_e": 4,
 LAYOUT_INTERIOR, LAYOUT_FACES, LAYOUT_HALO = 0, 1, 2
 OPT_EXACT, OPT_PATH, OPT_BLOCK, OPT_KERNEL, OPT_SYNC = 1, 2, 3, 4, 5
 TILES_WAVE_LOCAL, TILES_WAVE_LOCAL_1Z, TILES_PAIRS = 0, 1, 2
-EQ_LAUNCHES, EQ_TASKS = 1, 2
+EQ_LAUNCHES, EQ_TASKS, EQ_X_HANDOVER = 1, 2, 3
 Q_HOMOGENEOUS, Q_DEVICE_BYTES, Q_CELLS, Q_STEPS, Q_BYTES_PER_CELL, Q_KERNEL, Q_UNIFORM_XY = 1, 2, 3, 4, 5, 6, 7
 Q_X_HANDOVER = 8
 
@@ -424,7 +424,8 @@ class Ensemble:
         return {"members": m.value, "cells": c.value, "steps": s.value}
 
     def query(self, what: int) -> int:
-        """rt6d_ensemble_query: EQ_LAUNCHES (kernel launches per step) or EQ_TASKS."""
+        """rt6d_ensemble_query: EQ_LAUNCHES (kernel launches per step), EQ_TASKS or
+        EQ_X_HANDOVER (members whose step passes x faces between lanes)."""
         v = C.c_longlong()
         _check(self.lib, self.lib.rt6d_ensemble_query(self._h, what, C.byref(v)), "rt6d_ensemble_query")
         return v.value

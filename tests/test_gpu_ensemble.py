@@ -172,3 +172,33 @@ def test_ensemble_rejects_bad_members(gpu_available):
         ens.step(2)   # empty ensemble: nothing to do
     for m in (m32, mex, bare):
         m.close()
+
+
+@pytest.mark.parametrize("persistent", [False, True], ids=["dispatch", "persistent"])
+def test_row_aligned_members_pass_x_faces_between_lanes(persistent, gpu_available):
+    """Binary64 wave-local members that can all deal whole rows to their
+    wavefronts take the x-face hand-over together (RT6D_EQ_X_HANDOVER); one
+    member that cannot (24 x 16: whole rows would take a fourth wavefront
+    per plane where 192 pairs fill three) turns it off for the ensemble, so
+    that a step stays one launch.  Every member equals its own run, bit for
+    bit, either way."""
+    same = [cases.member_spec(77, i, n=(24, 24, 8, 8, 8, 8)) for i in range(3)]
+    mixed = same + [cases.member_spec(77, 9, n=(24, 16, 8, 8, 8, 8))]
+    for specs, handover in ((same, 3), (mixed, 0)):
+        ms = [_model(s, rt6d.FP64) for s in specs]
+        try:
+            with rt6d.Ensemble(rt6d.FP64) as ens:
+                for m in ms:
+                    ens.add(m)
+                ens.set_tiles(rt6d.TILES_WAVE_LOCAL)
+                if persistent:
+                    ens.set_schedule(4, 1)
+                assert ens.query(rt6d.EQ_X_HANDOVER) == handover
+                assert ens.query(rt6d.EQ_LAUNCHES) == 1
+                ens.step(5)
+                got = [m.download("E") for m in ms]
+            for s, E in zip(specs, got):
+                assert np.array_equal(E, _alone(s, rt6d.FP64, 5)), s.n
+        finally:
+            for m in ms:
+                m.close()
