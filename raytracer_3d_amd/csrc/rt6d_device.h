@@ -120,16 +120,24 @@ __device__ __forceinline__ double sb_flux_exact(double v, double dt, double dxf,
     return v * (E1 + E0) * 0.5 - fabs(v) * ((1.0 - cr) + ucfl * cr) * Rj * 0.5;
 }
 
+// x ^ (y & 0x80000000) in one instruction: gfx950's three-input bitwise
+// operation with the truth table of s0 ^ (s1 & s2), indexed by
+// (s0 << 2) | (s1 << 1) | s2 (0xf0 ^ (0xcc & 0xaa) = 0x78)
+__device__ __forceinline__ uint32_t xor_sign_of(uint32_t x, uint32_t y)
+{
+    return __builtin_amdgcn_bitop3_b32(x, y, 0x80000000u, 0x78);
+}
+
 // a with its sign flipped where b is negative (a * sign(b), exact)
 __device__ __forceinline__ double flip_sign_by(double a, double b)
 {
     const uint64_t ua = __builtin_bit_cast(uint64_t, a), ub = __builtin_bit_cast(uint64_t, b);
-    return __builtin_bit_cast(double, ua ^ (ub & 0x8000000000000000ull));
+    const uint32_t hi = xor_sign_of((uint32_t)(ua >> 32), (uint32_t)(ub >> 32));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | (ua & 0xffffffffull));
 }
 __device__ __forceinline__ float flip_sign_by(float a, float b)
 {
-    const uint32_t ua = __builtin_bit_cast(uint32_t, a), ub = __builtin_bit_cast(uint32_t, b);
-    return __builtin_bit_cast(float, ua ^ (ub & 0x80000000u));
+    return __builtin_bit_cast(float, xor_sign_of(__builtin_bit_cast(uint32_t, a), __builtin_bit_cast(uint32_t, b)));
 }
 
 // a*b + c with one rounding, in T
