@@ -101,17 +101,19 @@ inline ColX2Cfg colx2_config(const Geo &g, int zc, int w, bool rowal)
     return cc;
 }
 
-// v of the lane below (lane i receives lane i-1's value; lane 0 receives 0)
+// v of the lane below (lane i receives lane i-1's value; lane 0 receives 0,
+// which callers replace).  The plain DPP move, with no old value to merge,
+// needs no copy of the destination first.
 __device__ __forceinline__ float lane_shr1(float v)
 {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138 /* wave_shr:1 */,
-                                                                 0xf, 0xf, false));
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x138 /* wave_shr:1 */, 0xf,
+                                                              0xf, true));
 }
 __device__ __forceinline__ double lane_shr1(double v)
 {
     const uint64_t u = __builtin_bit_cast(uint64_t, v);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, 0x138, 0xf, 0xf, false);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), 0x138, 0xf, 0xf, false);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)u, 0x138, 0xf, 0xf, true);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), 0x138, 0xf, 0xf, true);
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
@@ -452,7 +454,7 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
                         // (the same operands and coefficients, so the same bits)
                         F2 = sb_flux_c<T>(k2, e.a, e.b, R.a, R.b);
                         if (xlast) F2 = vb * e.b;
-                        F0 = lane_shr1(F2);
+                        F0 = lane_shr1(F2);   // lane 0 starts a row
                         if (xfirst) F0 = v0 * e.a;
                     } else {
                         F0 = sb_flux_c<T>(k0, L.a, L.b, e.a, e.b);
