@@ -272,6 +272,14 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
         for (int q = 0; q < 4; ++q) sy[q] = (ys[q] - 1) * nxh + (x0 >> 1) - pw + H;
     }
     const int sown = xyc - pw + H;
+    // XAL x neighbours, in elements of T over the slots: the pair left of
+    // the own one, and the pair right of it; at a row start the left read
+    // starts one element later, so that its second element is the own first
+    // cell (the zero-gradient ghost that face x reads), and at a row end the
+    // right read one element earlier, so that its first element is the own
+    // second cell (the elements beyond are read but not used: the boundary
+    // faces are v*E of the edge cell)
+    const int lel = 2 * sown - (xfirst ? 1 : 2), rel = 2 * sown + (xlast ? 1 : 2);
     auto poff = [&](int j, int mm) -> uint32_t { return base0 + (uint32_t)j * planeB + moff(mm); };
     auto ldm = [&](int j, int mm) -> Pair<T> {
         mm = mm < 1 ? 1 : (mm > nm ? nm : mm);
@@ -409,7 +417,11 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
             };
             if (ax) {
                 Pair<T> L, R;
-                if constexpr (WL) {
+                if constexpr (XAL) {
+                    const T *se = (const T *)sl;
+                    L = Pair<T>{se[lel], se[lel + 1]};
+                    R = Pair<T>{se[rel], se[rel + 1]};
+                } else if constexpr (WL) {
                     L = slot(sown - 1);
                     R = slot(sown + 1);
                 } else {
@@ -419,10 +431,11 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
                 if constexpr (EXACT) {
                     if (xfirst) L = Pair<T>{e.a, e.a};
                     if (xlast) R = Pair<T>{e.b, e.b};
-                } else {
+                } else if constexpr (!XAL) {
                     // the fast arithmetic replaces the boundary faces (F0 where
                     // xfirst, F2 where xlast) by v*E of the edge cell, so only
-                    // the ghosts that face 1 reads are needed
+                    // the ghosts that face 1 reads are needed (the XAL reads
+                    // above take them from the own pair)
                     if (xfirst) L.b = e.a;
                     if (xlast) R.a = e.b;
                 }
