@@ -131,11 +131,12 @@ constexpr int kWLSlots = 128;
 // has the same dx), so that their coefficients are wave-uniform scalars
 // rather than per-lane values (fast arithmetic only).
 template <typename T, bool EXACT, int ZC, bool FULL, bool WL = false, bool UXY = false, bool MP = false,
-          bool XAL = false>
+          bool XAL = false, bool SB = false>
 __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, const ColX2Cfg &cc, uint32_t b)
 {
     static_assert(!(UXY && EXACT), "the exact arithmetic divides by the per-lane spacings");
     static_assert(!XAL || (WL && !EXACT), "the x-face hand-over needs row-aligned wave-local lanes");
+    static_assert(!SB || (WL && UXY && !EXACT), "the sign branch takes the face groups' coefficient sets");
     const uint32_t t = b * blockDim.x + threadIdx.x;
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2], nk = g.n[3], nl = g.n[4], nm = g.n[5];
     const int nxy = nx * ny, nxh = nx >> 1;
@@ -379,8 +380,11 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
                     FzB[i] = face_flux<T, EXACT>(v, dtv, dzf, czf, zbv[i], zbv[i + 1], zbv[i + 2], zbv[i + 3]);
                 } else {
                     const SbCoef<T> k = i < ZC ? rk(i, kRecZlo, kPosZlo) : rk(ZC - 1, kRecZhi, kPosZhi);
-                    FzA[i] = sb_flux_c<T>(k, za[i], za[i + 1], za[i + 2], za[i + 3]);
-                    FzB[i] = sb_flux_c<T>(k, zbv[i], zbv[i + 1], zbv[i + 2], zbv[i + 3]);
+                    on_sign<SB>(k.pos, [&](auto sg) {
+                        using S = decltype(sg);
+                        FzA[i] = sb_flux_s<T, S>(k, za[i], za[i + 1], za[i + 2], za[i + 3]);
+                        FzB[i] = sb_flux_s<T, S>(k, zbv[i], zbv[i + 1], zbv[i + 2], zbv[i + 3]);
+                    });
                 }
             }
         }
@@ -460,18 +464,21 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
                         k1 = sb_coef<T>(vi, (T)cx1);
                         k2 = sb_coef<T>(vi, (T)cx2);
                     }
-                    F1 = sb_flux_c<T>(k1, L.b, e.a, e.b, R.a);
+                    // the three faces share the sign of vi
+                    on_sign<SB>(k1.pos, [&](auto sg) {
+                        using S = decltype(sg);
+                        F1 = sb_flux_s<T, S>(k1, L.b, e.a, e.b, R.a);
+                        if constexpr (!XAL) F0 = sb_flux_s<T, S>(k0, L.a, L.b, e.a, e.b);
+                        F2 = sb_flux_s<T, S>(k2, e.a, e.b, R.a, R.b);
+                    });
                     if constexpr (XAL) {
                         // face x-1 is face x+1 of the pair to the left, held by
                         // the lane below, the wavefronts starting at row starts
                         // (the same operands and coefficients, so the same bits)
-                        F2 = sb_flux_c<T>(k2, e.a, e.b, R.a, R.b);
                         if (xlast) F2 = vb * e.b;
                         F0 = lane_shr1(F2);   // lane 0 starts a row
                         if (xfirst) F0 = v0 * e.a;
                     } else {
-                        F0 = sb_flux_c<T>(k0, L.a, L.b, e.a, e.b);
-                        F2 = sb_flux_c<T>(k2, e.a, e.b, R.a, R.b);
                         if (xfirst) F0 = v0 * e.a;
                         if (xlast) F2 = vb * e.b;
                     }
@@ -511,10 +518,13 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
                         kl = sb_coef<T>(vi, (T)cyl);
                         kr = sb_coef<T>(vi, (T)cyr);
                     }
-                    FLa = sb_flux_c<T>(kl, a0.a, a1.a, e.a, a3.a);
-                    FRa = sb_flux_c<T>(kr, a1.a, e.a, a3.a, a4.a);
-                    FLb = sb_flux_c<T>(kl, a0.b, a1.b, e.b, a3.b);
-                    FRb = sb_flux_c<T>(kr, a1.b, e.b, a3.b, a4.b);
+                    on_sign<SB>(kl.pos, [&](auto sg) {   // both faces have the sign of vi
+                        using S = decltype(sg);
+                        FLa = sb_flux_s<T, S>(kl, a0.a, a1.a, e.a, a3.a);
+                        FRa = sb_flux_s<T, S>(kr, a1.a, e.a, a3.a, a4.a);
+                        FLb = sb_flux_s<T, S>(kl, a0.b, a1.b, e.b, a3.b);
+                        FRb = sb_flux_s<T, S>(kr, a1.b, e.b, a3.b, a4.b);
+                    });
                     // boundary rows: only wavefronts that hold row 1 or row ny
                     // (a wave-uniform branch)
                     if (ybw) {
@@ -548,8 +558,11 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
                     FRb = face_flux<T, EXACT>(vR, dtv, dm, cm, wm1[j].b, e.b, wp1[j].b, wp2[j].b);
                 } else {
                     const SbCoef<T> k = rk(j, kRecMhi, kPosMhi);
-                    FRa = sb_flux_c<T>(k, wm1[j].a, e.a, wp1[j].a, wp2[j].a);
-                    FRb = sb_flux_c<T>(k, wm1[j].b, e.b, wp1[j].b, wp2[j].b);
+                    on_sign<SB>(k.pos, [&](auto sg) {
+                        using S = decltype(sg);
+                        FRa = sb_flux_s<T, S>(k, wm1[j].a, e.a, wp1[j].a, wp2[j].a);
+                        FRb = sb_flux_s<T, S>(k, wm1[j].b, e.b, wp1[j].b, wp2[j].b);
+                    });
                 }
                 if constexpr (EXACT) {
                     const double dm = sld(p.ax.dx[5], m);
@@ -601,12 +614,12 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
 }
 
 template <typename T, bool EXACT, int ZC, bool FULL, bool WL = false, bool UXY = false, bool MP = false,
-          bool XAL = false>
+          bool XAL = false, bool SB = false>
 __global__ void __launch_bounds__(256) k_step_colx2(const Geo g, const StepPtrs<T> p, const ColX2Cfg cc, uint32_t nbl)
 {
     const uint32_t b = xcd_logical_block(blockIdx.x, nbl);
     if (b >= nbl) return;
-    colx2_body<T, EXACT, ZC, FULL, WL, UXY, MP, XAL>(g, p, cc, b);
+    colx2_body<T, EXACT, ZC, FULL, WL, UXY, MP, XAL, SB>(g, p, cc, b);
 }
 
 // The same kernel held to 128 VGPRs, four wavefronts per SIMD: the renamed

@@ -194,6 +194,57 @@ __device__ __forceinline__ T sb_flux_c(const SbCoef<T> &k, T Em1, T E0, T E1, T 
     return fma_t<T>(k.w, phiR, fma_t<T>(k.vp, E0, k.vn * E1));
 }
 
+// The same flux with the upwind side fixed (POS: v > 0): Ru without a select,
+// and v*E_up as one product.  Where v*E_up is not zero it equals
+// fma(vp, E0, vn*E1) of sb_flux_c exactly (the other product is a zero); a
+// zero product may differ in its sign only, which the sums that follow do
+// not see (E + dt*dE is the same).
+template <typename T, bool POS>
+__device__ __forceinline__ T sb_flux_up(const SbCoef<T> &k, T Em1, T E0, T E1, T E2)
+{
+    const T R = E1 - E0;
+    const T Ru = POS ? (E0 - Em1) : (E2 - E1);
+    const T su = flip_sign_by(Ru, R);
+    const T aR = fabs(R);
+    const T mn = fmin(aR, su), mx = fmax(aR, su);
+    T p;
+    if constexpr (sizeof(T) == 4) p = __builtin_amdgcn_fmed3f(mn + mn, T(0), mx);   // mx >= 0
+    else p = fmax(fmin(mn + mn, mx), T(0));
+    const T phiR = __builtin_copysign(p, R);
+    return fma_t<T>(k.w, phiR, POS ? k.vp * E0 : k.vn * E1);
+}
+
+// The upwind side of a group of faces that share the sign of v: fixed at
+// compile time on each side of a wave-uniform branch (SignC<true/false>), or
+// selected per face (SignSel).
+template <bool B>
+struct SignC {
+    static constexpr bool sel = false, value = B;
+};
+struct SignSel {
+    static constexpr bool sel = true, value = false;
+};
+
+template <typename T, typename S>
+__device__ __forceinline__ T sb_flux_s(const SbCoef<T> &k, T Em1, T E0, T E1, T E2)
+{
+    if constexpr (S::sel) return sb_flux_c<T>(k, Em1, E0, E1, E2);
+    else return sb_flux_up<T, S::value>(k, Em1, E0, E1, E2);
+}
+
+// f(S) for faces whose v has the sign of pos: with BR a branch on the sign
+// and the upwind side fixed on each side, else the per-face select.
+template <bool BR, typename F>
+__device__ __forceinline__ void on_sign(bool pos, F &&f)
+{
+    if constexpr (BR) {
+        if (pos) f(SignC<true>{});
+        else f(SignC<false>{});
+    } else {
+        f(SignSel{});
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ T sb_flux_fast(T v, T cfl_coef, T Em1, T E0, T E1, T E2)
 {

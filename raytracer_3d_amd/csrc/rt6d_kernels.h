@@ -571,12 +571,13 @@ hipError_t launch_colx2(const Geo &g, const StepPtrs<T> &p, hipStream_t stream)
                     hipLaunchKernelGGL((k_step_colx2_w4<T, EXACT, ZC, true, WL, true>), dim3(nbh), dim3(256), 0,
                                        stream, g, p, cc, nbl);
             } else {
+                constexpr bool SB = ZC == 2;
                 if (cc.xal)
-                    hipLaunchKernelGGL((k_step_colx2<T, EXACT, ZC, true, WL, true, false, true>), dim3(nbh), dim3(256),
-                                       0, stream, g, p, cc, nbl);
+                    hipLaunchKernelGGL((k_step_colx2<T, EXACT, ZC, true, WL, true, false, true, SB>), dim3(nbh),
+                                       dim3(256), 0, stream, g, p, cc, nbl);
                 else
-                    hipLaunchKernelGGL((k_step_colx2<T, EXACT, ZC, true, WL, true>), dim3(nbh), dim3(256), 0, stream,
-                                       g, p, cc, nbl);
+                    hipLaunchKernelGGL((k_step_colx2<T, EXACT, ZC, true, WL, true, false, false, SB>), dim3(nbh),
+                                       dim3(256), 0, stream, g, p, cc, nbl);
             }
             return hipGetLastError();
         }
