@@ -130,6 +130,13 @@ constexpr int kWLSlots = 128;
 // UXY: the x and y spacings are uniform (every face and cell of both axes
 // has the same dx), so that their coefficients are wave-uniform scalars
 // rather than per-lane values (fast arithmetic only).
+// XAL (with WL, fast arithmetic): the wavefronts start at row starts
+// (ColX2Cfg::xal), so the x face between two pairs is computed once, by the
+// lane on its left, and passed to the lane on its right (lane_shr1), and the
+// row-end lanes read their zero-gradient ghosts out of their own slot.
+// SB (with WL and UXY): each face group branches on the sign of its velocity,
+// which is uniform over the wavefront (on_sign), instead of selecting the
+// upwind side per face.
 template <typename T, bool EXACT, int ZC, bool FULL, bool WL = false, bool UXY = false, bool MP = false,
           bool XAL = false, bool SB = false>
 __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, const ColX2Cfg &cc, uint32_t b)
