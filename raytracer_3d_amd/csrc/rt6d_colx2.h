@@ -315,8 +315,12 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
     Pair<T>(&w0i)[ZC] = win[1];
     Pair<T>(&wp1i)[ZC] = win[2];
     Pair<T> zq[4];
+    // lanes without a halo pair never store theirs: the XAL kernels leave it
+    // unset (no zeroing per step); the others zero it, which measured fewer
+    // registers there
     auto ldhal = [&](int j, int mm) -> Pair<T> {
-        Pair<T> h{T(0), T(0)};
+        Pair<T> h;
+        if constexpr (!XAL) h = Pair<T>{T(0), T(0)};
         if (hval) h = bld2<T>(rin(mm), ohal, poff(j, mm));
         return h;
     };
