@@ -140,6 +140,11 @@ __device__ __forceinline__ float flip_sign_by(float a, float b)
     return __builtin_bit_cast(float, xor_sign_of(__builtin_bit_cast(uint32_t, a), __builtin_bit_cast(uint32_t, b)));
 }
 
+// a with the sign of b, in T (the generic builtin is binary64: a binary32
+// operand would be widened first)
+__device__ __forceinline__ double copysign_t(double a, double b) { return __builtin_copysign(a, b); }
+__device__ __forceinline__ float copysign_t(float a, float b) { return __builtin_copysignf(a, b); }
+
 // a*b + c with one rounding, in T
 template <typename T>
 __device__ __forceinline__ T fma_t(T a, T b, T c)
@@ -190,7 +195,7 @@ __device__ __forceinline__ T sb_flux_c(const SbCoef<T> &k, T Em1, T E0, T E1, T 
     T p;
     if constexpr (sizeof(T) == 4) p = __builtin_amdgcn_fmed3f(mn + mn, T(0), mx);   // mx >= 0
     else p = fmax(fmin(mn + mn, mx), T(0));
-    const T phiR = __builtin_copysign(p, R);
+    const T phiR = copysign_t(p, R);
     return fma_t<T>(k.w, phiR, fma_t<T>(k.vp, E0, k.vn * E1));
 }
 
@@ -210,7 +215,7 @@ __device__ __forceinline__ T sb_flux_up(const SbCoef<T> &k, T Em1, T E0, T E1, T
     T p;
     if constexpr (sizeof(T) == 4) p = __builtin_amdgcn_fmed3f(mn + mn, T(0), mx);   // mx >= 0
     else p = fmax(fmin(mn + mn, mx), T(0));
-    const T phiR = __builtin_copysign(p, R);
+    const T phiR = copysign_t(p, R);
     return fma_t<T>(k.w, phiR, POS ? k.vp * E0 : k.vn * E1);
 }
 
