@@ -213,7 +213,8 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU only: the 6D step runs one resident grid per GPU and this build has "
-              "no multi-GPU path (see DESIGN.md for how it would shard); run with --gpus 1.", file=sys.stderr)
+              "no multi-GPU path (a sharded step would split m into one slab per GPU with a 2-plane halo exchange); "
+              "run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     from raytracer_3d_amd import cases, rt6d
