@@ -159,12 +159,20 @@ def _step_kernel(precision, option, uxy, handover=False):
     handover: RT6D_Q_X_HANDOVER (the row-aligned wave-local kernel)."""
     T = precision
     # binary64's uniform-spacing wave-local kernels are the 128-VGPR entry
-    k = "k_step_colx2_w4" if (uxy and T == "double") else "k_step_colx2"
-    wl = f"{k}<{T}, false, {{}}, true, true, {'true' if uxy else 'false'}, false, {'true' if handover else 'false'}>"
+    # (eight template parameters); the others carry a ninth, SB: binary32's
+    # wave-local kernel with two z cells per thread branches on face signs
+    w4 = uxy and T == "double"
+    k = "k_step_colx2_w4" if w4 else "k_step_colx2"
+
+    def wl(zc):
+        sb = "" if w4 else (", true" if (uxy and T == "float" and zc == 2) else ", false")
+        return (f"{k}<{T}, false, {zc}, true, true, {'true' if uxy else 'false'}, false, "
+                f"{'true' if handover else 'false'}{sb}>")
+
     names = {
-        15: f"k_step_colx2<{T}, false, 1, true, false, false, false, false>",
-        16: f"k_step_colx2<{T}, false, 2, true, false, false, false, false>",
-        21: wl.format(1), 22: wl.format(2),
+        15: f"k_step_colx2<{T}, false, 1, true, false, false, false, false, false>",
+        16: f"k_step_colx2<{T}, false, 2, true, false, false, false, false, false>",
+        21: wl(1), 22: wl(2),
         18: "k_step_colq<false, 1, true>", 19: "k_step_colq<false, 2, true>",
         9: f"k_step_col<{T}, false, false, 1, true, 1>", 10: f"k_step_col<{T}, false, false, 2, true, 1>",
         4: f"k_step_col<{T}, false, true, 1, true, 0>", 5: f"k_step_col<{T}, false, true, 2, true, 0>",
