@@ -538,7 +538,7 @@ inline bool colx2_hands_over(const Geo &g, bool exact, int zc)
 }
 
 template <typename T, bool EXACT, int ZC, bool WL = false>
-hipError_t launch_colx2(const Geo &g, const StepPtrs<T> &p, hipStream_t stream)
+hipError_t launch_colx2(const Geo &g, const StepPtrs<T> &p, int ncu, hipStream_t stream)
 {
     // the wave-local kernels in fast arithmetic take uniform x and y spacings
     // as scalars; where the wavefronts start at row starts they also hand
@@ -547,8 +547,11 @@ hipError_t launch_colx2(const Geo &g, const StepPtrs<T> &p, hipStream_t stream)
     // 16^3 0.2669 against 0.2725 ms); binary32 only where rows align anyway
     // (the 60-pair wavefronts of 24^3 measured 0.1485 against 0.1464 ms).
     const bool uxyw = WL && !EXACT && (g.uxy & 3) == 3 && is_full<true>(g);
-    const ColX2Cfg cc = colx2_config(g, ZC, 2, WL && sizeof(T) == 8 && colx2_rowal<T>(g, EXACT, ZC));
+    ColX2Cfg cc = colx2_config(g, ZC, 2, WL && sizeof(T) == 8 && colx2_rowal<T>(g, EXACT, ZC));
     const uint64_t nthreads = (uint64_t)cc.nxyp * cc.nzb * g.n[3] * g.n[4];
+    // the binary32 sign-branch kernel gates its far outer z loads on the
+    // face signs where the grid fills the device (32 wavefronts per CU)
+    cc.fz = nthreads / 64u >= (uint64_t)ncu * 32u ? 1 : 0;
     const uint32_t nbl = (uint32_t)((nthreads + 255) / 256);
     const uint32_t nbh = ((nbl + 7u) / 8u) * 8u;
     // E beyond the reach of 32-bit offsets: addressed one m plane at a time
@@ -728,12 +731,12 @@ hipError_t launch_step_impl(const Geo &g, const StepPtrs<T> &p, bool homog, cons
     case 9: return launch_col<T, EXACT, 1, 1>(g, p, homog, lc.stream);
     case 10: return launch_col<T, EXACT, 2, 1>(g, p, homog, lc.stream);
     case 11: return launch_col<T, EXACT, 4, 1>(g, p, homog, lc.stream);
-    case 15: return launch_colx2<T, EXACT, 1>(g, p, lc.stream);
-    case 16: return launch_colx2<T, EXACT, 2>(g, p, lc.stream);
-    case 17: return launch_colx2<T, EXACT, 4>(g, p, lc.stream);
-    case 21: return launch_colx2<T, EXACT, 1, true>(g, p, lc.stream);
-    case 22: return launch_colx2<T, EXACT, 2, true>(g, p, lc.stream);
-    case 23: return launch_colx2<T, EXACT, 4, true>(g, p, lc.stream);
+    case 15: return launch_colx2<T, EXACT, 1>(g, p, lc.ncu, lc.stream);
+    case 16: return launch_colx2<T, EXACT, 2>(g, p, lc.ncu, lc.stream);
+    case 17: return launch_colx2<T, EXACT, 4>(g, p, lc.ncu, lc.stream);
+    case 21: return launch_colx2<T, EXACT, 1, true>(g, p, lc.ncu, lc.stream);
+    case 22: return launch_colx2<T, EXACT, 2, true>(g, p, lc.ncu, lc.stream);
+    case 23: return launch_colx2<T, EXACT, 4, true>(g, p, lc.ncu, lc.stream);
     default: break;
     }
     if constexpr (sizeof(T) == 4) {

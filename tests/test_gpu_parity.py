@@ -364,6 +364,11 @@ def test_full_size_fp32_form(gm_full, gpu_available):
         m.integrate(20, dt=dt)
         e32 = m.download("E")
     assert rel_l2(e32, e64) <= TOL32
+    # at this size the default binary32 kernel loads the far outer z planes
+    # only where the face signs read them; the column kernel loads all four
+    with _model(c, precision=rt6d.FP32, kernel=5) as m:
+        m.integrate(20, dt=dt)
+        assert np.array_equal(m.download("E"), e32)
 
 
 # ----------------------------------------------------------- config-0 size
