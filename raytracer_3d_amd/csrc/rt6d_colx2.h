@@ -378,7 +378,8 @@ __device__ __forceinline__ void colx2_body(const Geo &g, const StepPtrs<T> &p, c
             // grid fills the device (binary32 at 24^3 x 16^3: 0.1336 instead
             // of 0.1394 ms, 32^3 0.2829 instead of 0.2908, E bitwise the
             // same) and costs on smaller grids, which keep all four loads
-            // (ColX2Cfg::fz; 16^3 x 12^3 gated: 0.0311 against 0.0289 ms)
+            // (ColX2Cfg::fz; 16^3 x 12^3 with the gate on: 0.0311 against
+            // 0.0289 ms)
             auto flg = [&](int j) -> uint32_t {
                 const T fv = rv(j, kRecFlags);
                 if constexpr (sizeof(T) == 8) return (uint32_t)__builtin_bit_cast(uint64_t, fv);

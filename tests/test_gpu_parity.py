@@ -6,6 +6,7 @@
 * the C oracle at medium size over many steps;
 * size-independent properties at the 24^3 x 16^3 production size.
 """
+import copy
 import math
 
 import numpy as np
@@ -165,10 +166,11 @@ def test_medium_packet_many_steps_against_oracle(exact, gpu_available):
 
 
 def test_fp32_default_kernel_choice_is_invisible(gpu_available):
-    """The binary32 homogeneous default is the quad kernel (four x and two z
-    cells per thread) where nx is divisible by 4, nz even and the grid fills
-    the device, else the pair kernel; all of them give the column kernel's
-    values bit for bit, so ensembles and single contexts agree."""
+    """The binary32 homogeneous default is the wave-local pair kernel where nx
+    is even and at most 32 (the plain pair kernel on grids too small to fill
+    the device with one z cell per thread, the quad kernel for wider rows);
+    all of them give the column kernel's values bit for bit, so ensembles
+    and single contexts agree."""
     case = cases.packet_case(n=(16, 8, 8, 4, 4, 4), name="fp32_pairs")   # 64 pairs per plane
     outs = {}
     for kernel in (0, 5, 16, 17, 19, 21, 22, 23):
@@ -366,9 +368,18 @@ def test_full_size_fp32_form(gm_full, gpu_available):
     assert rel_l2(e32, e64) <= TOL32
     # at this size the default binary32 kernel loads the far outer z planes
     # only where the face signs read them; the column kernel loads all four
+    # (with the z reflection and with zero-gradient z ends)
     with _model(c, precision=rt6d.FP32, kernel=5) as m:
         m.integrate(20, dt=dt)
         assert np.array_equal(m.download("E"), e32)
+    c_open = copy.copy(c)
+    c_open.reflect_up = c_open.reflect_down = False
+    outs = []
+    for kernel in (0, 5):
+        with _model(c_open, precision=rt6d.FP32, kernel=kernel) as m:
+            m.integrate(20, dt=dt)
+            outs.append(m.download("E"))
+    assert np.array_equal(outs[0], outs[1])
 
 
 # ----------------------------------------------------------- config-0 size
